@@ -1,0 +1,105 @@
+"""ctypes binding of libgatlas_hip.so (declarations in include/gatlas.h).
+
+Loading fails loudly when the shared library is missing: there is no CPU
+fallback for any entry point.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgatlas_hip.so")
+
+GAT_F32, GAT_F64, GAT_BF16, GAT_BF16X3, GAT_FP8 = 0, 1, 2, 3, 4
+GAT_MATMUL, GAT_MATVEC, GAT_SAXPY = 0, 1, 2
+GAT_BUFFER, GAT_IMAGE, GAT_MFMA, GAT_NATIVE = 0, 1, 2, 3
+
+_c_sz = ctypes.c_size_t
+_c_vp = ctypes.c_void_p
+_c_int = ctypes.c_int
+_c_dbl = ctypes.c_double
+_c_u64 = ctypes.c_uint64
+
+
+class TileCfg(ctypes.Structure):
+    _fields_ = [("waves_m", _c_int), ("waves_n", _c_int), ("tm", _c_int), ("tn", _c_int),
+                ("bk", _c_int), ("blk", _c_int)]
+
+    def as_tuple(self):
+        return (self.waves_m, self.waves_n, self.tm, self.tn, self.bk, self.blk)
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "gat_create": (_c_vp, [_c_int]),
+    "gat_destroy": (None, [_c_vp]),
+    "gat_last_error": (ctypes.c_char_p, []),
+    "gat_max_workgroup": (_c_sz, [_c_vp]),
+    "gat_device_info": (_c_int, [_c_vp, ctypes.c_char_p, _c_sz]),
+    "gat_stream": (_c_vp, [_c_vp]),
+    "gat_device_count": (_c_int, []),
+    "gat_malloc": (_c_int, [_c_vp, _c_sz, ctypes.POINTER(_c_vp)]),
+    "gat_free": (_c_int, [_c_vp, _c_vp]),
+    "gat_memcpy_h2d": (_c_int, [_c_vp, _c_vp, _c_vp, _c_sz]),
+    "gat_memcpy_d2h": (_c_int, [_c_vp, _c_vp, _c_vp, _c_sz]),
+    "gat_memcpy_d2d": (_c_int, [_c_vp, _c_vp, _c_vp, _c_sz]),
+    "gat_memset": (_c_int, [_c_vp, _c_vp, _c_int, _c_sz]),
+    "gat_sync": (_c_int, [_c_vp]),
+    "gat_timer_start": (_c_int, [_c_vp]),
+    "gat_timer_stop": (_c_int, [_c_vp, ctypes.POINTER(ctypes.c_float)]),
+    "gat_fill_uniform": (_c_int, [_c_vp, _c_int, _c_vp, _c_sz, _c_u64, _c_u64, _c_dbl, _c_dbl]),
+    "gat_fill_normal": (_c_int, [_c_vp, _c_int, _c_vp, _c_sz, _c_u64, _c_u64]),
+    "gat_scale_columns": (_c_int, [_c_vp, _c_vp, _c_sz, _c_sz, _c_u64, _c_u64]),
+    "gat_gemm": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, _c_dbl, _c_vp, _c_sz, _c_vp,
+                          _c_sz, _c_dbl, _c_vp, _c_sz]),
+    "gat_gemm_strided": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, _c_dbl, _c_vp, _c_sz,
+                                  _c_sz, _c_vp, _c_sz, _c_sz, _c_dbl, _c_vp, _c_sz, _c_sz, _c_sz]),
+    "gat_gemm_cfg": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, _c_dbl, _c_vp, _c_sz,
+                              _c_vp, _c_sz, _c_dbl, _c_vp, _c_sz, ctypes.POINTER(TileCfg)]),
+    "gat_tile_cfg_count": (_c_int, [_c_int]),
+    "gat_tile_cfg_get": (_c_int, [_c_int, _c_int, ctypes.POINTER(TileCfg)]),
+    "gat_gemm_compat": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, _c_int, _c_dbl, _c_vp,
+                                 _c_vp, _c_dbl, _c_vp, _c_sz, _c_sz, _c_sz, _c_sz]),
+    "gat_gemv": (_c_int, [_c_vp, _c_int, _c_int, _c_sz, _c_sz, _c_dbl, _c_vp, _c_sz, _c_vp, _c_dbl, _c_vp]),
+    "gat_gemv_compat": (_c_int, [_c_vp, _c_int, _c_int, _c_sz, _c_sz, _c_int, _c_dbl, _c_vp, _c_vp, _c_dbl,
+                                 _c_vp, _c_sz, _c_sz, _c_sz, _c_sz]),
+    "gat_axpy": (_c_int, [_c_vp, _c_int, _c_sz, _c_dbl, _c_vp, _c_vp, _c_vp]),
+    "gat_axpy_compat": (_c_int, [_c_vp, _c_int, _c_sz, _c_sz, _c_dbl, _c_vp, _c_vp, _c_vp, _c_sz, _c_sz,
+                                 _c_sz, _c_sz, _c_sz, _c_sz]),
+}
+
+_lib = None
+
+
+class GatError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libgatlas_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libgatlas_hip.so is not built (expected at %s); run __graft_entry__.build() or "
+            "make -C gatlas_amd/csrc" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error():
+    msg = load().gat_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise GatError("%s failed (%d): %s" % (what, rc, last_error()))
+    return rc

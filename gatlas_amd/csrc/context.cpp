@@ -1,0 +1,185 @@
+#include "context.hpp"
+
+#include <cstdio>
+#include <sstream>
+
+namespace gat {
+
+static int ilog2_clamp(int64_t v, int lo, int hi) {
+    int l = 0;
+    while ((int64_t(1) << (l + 1)) <= v) ++l;
+    if (l < lo) l = lo;
+    if (l > hi) l = hi;
+    return l;
+}
+
+BucketKey bucket_of(int dtype, const GemmArgs& a) {
+    return BucketKey{dtype, a.transA, a.transB, ilog2_clamp(a.M, 4, 15), ilog2_clamp(a.N, 4, 15),
+                     ilog2_clamp(a.K, 4, 15)};
+}
+
+Context::~Context() {
+    if (device < 0) return;
+    hipSetDevice(device);
+    if (stream) hipStreamSynchronize(stream);
+    for (auto& kv : owned_) hipFree(kv.first);
+    owned_.clear();
+    if (work_) hipFree(work_);
+    if (ev_start) hipEventDestroy(ev_start);
+    if (ev_stop) hipEventDestroy(ev_stop);
+    if (stream) hipStreamDestroy(stream);
+}
+
+int Context::init(int dev) {
+    int n = 0;
+    GAT_HIP_TRY(hipGetDeviceCount(&n));
+    if (dev < 0 || dev >= n) {
+        set_error("gat_create: device index out of range");
+        return ST_INVALID;
+    }
+    device = dev;
+    GAT_HIP_TRY(hipSetDevice(dev));
+    hipDeviceProp_t p;
+    GAT_HIP_TRY(hipGetDeviceProperties(&p, dev));
+    max_workgroup = p.maxThreadsPerBlock;
+    num_cu = p.multiProcessorCount;
+    total_mem = p.totalGlobalMem;
+    arch = p.gcnArchName;
+    // Default-priority, non-blocking stream (the reference's per-device queue).
+    GAT_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    GAT_HIP_TRY(hipEventCreate(&ev_start));
+    GAT_HIP_TRY(hipEventCreate(&ev_stop));
+    return ST_OK;
+}
+
+void Context::make_current() const { hipSetDevice(device); }
+
+std::string Context::info() const {
+    std::ostringstream os;
+    os << "device " << device << " arch " << arch << " CUs " << num_cu << " maxWorkGroup " << max_workgroup
+       << " globalMem " << (total_mem >> 20) << " MiB";
+    return os.str();
+}
+
+int Context::alloc(size_t bytes, void** p) {
+    if (!p) return ST_INVALID;
+    make_current();
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return ST_NOMEM;
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    owned_[d] = bytes;
+    *p = d;
+    return ST_OK;
+}
+
+int Context::release(void* p) {
+    if (!p) return ST_OK;
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = owned_.find(p);
+        if (it == owned_.end()) {
+            set_error("gat_free: pointer not owned by this context");
+            return ST_INVALID;
+        }
+        owned_.erase(it);
+    }
+    make_current();
+    GAT_HIP_TRY(hipStreamSynchronize(stream));
+    GAT_HIP_TRY(hipFree(p));
+    return ST_OK;
+}
+
+int Context::workspace(size_t bytes, void** p) {
+    if (bytes > work_bytes_) {
+        make_current();
+        if (work_) {
+            GAT_HIP_TRY(hipStreamSynchronize(stream));
+            GAT_HIP_TRY(hipFree(work_));
+            work_ = nullptr;
+            work_bytes_ = 0;
+        }
+        hipError_t e = hipMalloc(&work_, bytes);
+        if (e != hipSuccess) {
+            set_error(std::string("workspace hipMalloc: ") + hipGetErrorString(e));
+            return ST_NOMEM;
+        }
+        work_bytes_ = bytes;
+    }
+    *p = work_;
+    return ST_OK;
+}
+
+void Context::set_tuned(const BucketKey& k, const MfmaCfg& c) {
+    std::lock_guard<std::mutex> g(mu_);
+    tuned_[k] = c;
+}
+
+bool Context::get_tuned(const BucketKey& k, MfmaCfg* c) const {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = tuned_.find(k);
+    if (it == tuned_.end()) return false;
+    *c = it->second;
+    return true;
+}
+
+void Context::clear_tuned() {
+    std::lock_guard<std::mutex> g(mu_);
+    tuned_.clear();
+}
+
+int Context::num_cfgs(int dtype) {
+    switch (dtype) {
+        case DT_F32: return mfma_f32_num_cfgs();
+        case DT_F64: return mfma_f64_num_cfgs();
+    }
+    return 0;
+}
+
+MfmaCfg Context::cfg_at(int dtype, int i) {
+    switch (dtype) {
+        case DT_F32: return mfma_f32_cfg(i);
+        case DT_F64: return mfma_f64_cfg(i);
+    }
+    return MfmaCfg{0, 0, 0, 0, 0, 0};
+}
+
+bool Context::has_cfg(int dtype, const MfmaCfg& c) {
+    switch (dtype) {
+        case DT_F32: return mfma_f32_has(c);
+        case DT_F64: return mfma_f64_has(c);
+    }
+    return false;
+}
+
+MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
+    const int64_t mn = a.M < a.N ? a.M : a.N;
+    if (dtype == DT_F64) {
+        if (mn >= 1024) return MfmaCfg{2, 2, 4, 4, 16, 16};
+        return MfmaCfg{2, 2, 2, 2, 16, 16};
+    }
+    // FP32 (and the FP32-output forms): large square-ish -> 128x128 tiles.
+    if (mn >= 1024) return MfmaCfg{2, 2, 2, 2, 16, 32};
+    if (mn >= 256) return MfmaCfg{2, 2, 1, 1, 32, 32};
+    return MfmaCfg{1, 1, 2, 2, 32, 32};
+}
+
+int Context::gemm(int dtype, const GemmArgs& a, const MfmaCfg* cfg) {
+    MfmaCfg c;
+    if (cfg) {
+        c = *cfg;
+    } else if (!get_tuned(bucket_of(dtype, a), &c)) {
+        c = default_cfg(dtype, a);
+    }
+    switch (dtype) {
+        case DT_F32: return launch_gemm_mfma_f32(c, a, stream);
+        case DT_F64: return launch_gemm_mfma_f64(c, a, stream);
+    }
+    set_error("gemm: precision form not available");
+    return ST_UNSUPPORTED;
+}
+
+}  // namespace gat

@@ -1,0 +1,88 @@
+// Launcher declarations for every hand-written gfx950 kernel of the library.
+#pragma once
+
+#include "common.hpp"
+
+namespace gat {
+
+// One GEMM problem, row-major storage as the reference lays it out
+// (KernelBaseMatmul.hpp:59-80 MatmulDataLayout: transposed == column-major):
+//   op(A)[i][k] = transA ? A[k*lda + i] : A[i*lda + k]
+//   op(B)[k][j] = transB ? B[j*ldb + k] : B[k*ldb + j]
+//   C[i*ldc + j] = alpha * sum_k op(A)[i][k] op(B)[k][j] + beta * C[i*ldc + j]
+// `batch` same-shape problems follow each other at the given strides
+// (the reference's packedCalc, KernelMatmulBuffer.hpp:229-230).
+// beta == 0 never reads C.
+struct GemmArgs {
+    int transA = 0, transB = 0;
+    int64_t M = 0, N = 0, K = 0;
+    const void* A = nullptr;
+    int64_t lda = 0, strideA = 0;
+    const void* B = nullptr;
+    int64_t ldb = 0, strideB = 0;
+    void* C = nullptr;
+    int64_t ldc = 0, strideC = 0;
+    double alpha = 1.0, beta = 0.0;
+    int batch = 1;
+    // FP8 only: per-128x128-block FP32 scales of op(A) (ceil(M/128) x ceil(K/128),
+    // row-major) and op(B) (ceil(K/128) x ceil(N/128), row-major).
+    const float* scaleA = nullptr;
+    const float* scaleB = nullptr;
+};
+
+// MFMA tile configuration.  The work-group is waves_m x waves_n wavefronts;
+// each wave owns tm x tn MFMA output blocks of `blk` x `blk` (32 or 16);
+// the macro tile is (waves_m*tm*blk) x (waves_n*tn*blk) and the LDS K depth bk.
+struct MfmaCfg {
+    int waves_m, waves_n, tm, tn, bk, blk;
+};
+
+// Reference-geometry ("compat") GEMM: work-group gs x gs work-items, each
+// owning bh consecutive rows x vl consecutive columns, K staged through LDS
+// gs*vl at a time (KernelMatmulBuffer.hpp:174-342).
+struct CompatCfg {
+    int gs, bh, vl;
+};
+
+// ---- fill / convert (fill.hip, convert.hip) ----
+int launch_fill_uniform(int dtype, void* out, size_t n, uint64_t seed, uint64_t stream, double lo,
+                        double hi, hipStream_t st);
+int launch_fill_normal(int dtype, void* out, size_t n, uint64_t seed, uint64_t stream, hipStream_t st);
+int launch_scale_columns(float* a, size_t rows, size_t cols, uint64_t seed, uint64_t stream,
+                         hipStream_t st);
+
+// ---- SAXPY (saxpy.hip) ----
+// z = alpha*x + y, n elements, separate multiply and add roundings.
+int launch_saxpy_native(int dtype, size_t n, double alpha, const void* x, const void* y, void* z,
+                        hipStream_t st);
+// Reference geometry (KernelSaxpyBuffer.hpp:143-225): 1-D when N == vl.
+int launch_saxpy_compat(int dtype, int vl, size_t M, size_t N, size_t gH, size_t gW, size_t bh,
+                        size_t bw, int packed, double alpha, const void* x, const void* y, void* z,
+                        hipStream_t st);
+
+// ---- GEMV (gemv.hip) ----
+// y = alpha*op(A)*x + beta*y.  Native: wavefront-cooperative rows (row-major)
+// or column-split partials with a deterministic second pass (transposed);
+// `work` must hold gemv_workspace_bytes() bytes.
+size_t gemv_workspace_bytes(int dtype, int transA, size_t M, size_t N);
+int launch_gemv_native(int dtype, int transA, size_t M, size_t N, double alpha, const void* A,
+                       size_t lda, const void* x, double beta, void* y, void* work, hipStream_t st);
+// Reference geometry (KernelMatvecBuffer.hpp:180-325): each work-item owns bh
+// rows, j-ascending unfused chain per row.
+int launch_gemv_compat(int dtype, int vl, int transA, size_t M, size_t N, size_t gs, size_t bh,
+                       int packed, int gemv, double alpha, double beta, const void* A, const void* x,
+                       void* y, hipStream_t st);
+
+// ---- GEMM ----
+int launch_gemm_compat(int dtype, const CompatCfg& cfg, const GemmArgs& a, hipStream_t st);
+int launch_gemm_mfma_f32(const MfmaCfg& cfg, const GemmArgs& a, hipStream_t st);
+int launch_gemm_mfma_f64(const MfmaCfg& cfg, const GemmArgs& a, hipStream_t st);
+bool mfma_f32_has(const MfmaCfg& cfg);
+bool mfma_f64_has(const MfmaCfg& cfg);
+int mfma_f32_num_cfgs();
+MfmaCfg mfma_f32_cfg(int i);
+int mfma_f64_num_cfgs();
+MfmaCfg mfma_f64_cfg(int i);
+bool compat_has(int dtype, const CompatCfg& cfg);
+
+}  // namespace gat

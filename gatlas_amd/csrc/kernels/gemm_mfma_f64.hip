@@ -1,0 +1,247 @@
+// FP64 GEMM on gfx950 matrix cores (v_mfma_f64_16x16x4_f64), kept FP64 end
+// to end.  Same work decomposition as the FP32 MFMA kernel: WM x WN waves,
+// each owning TM x TN 16x16 output blocks; K staged through double-buffered
+// LDS with both operands stored k-contiguous ([m][k] and [n][k], row stride
+// BK+2 doubles) so that one ds_read_b128 hands a lane the operands of two
+// MFMAs.  MFMA s of a k-group takes, in lane l, k = kk + 2*(l/16) + s.
+// Output block layout: lane l holds rows 4*(l/16) + r, r < 4, of column l%16.
+#include "../kernels.hpp"
+
+namespace gat {
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void tile_coords_f64(int wg, int nwg, int tiles_m, int tiles_n, int group_m,
+                                                int& tm, int& tn) {
+    if ((nwg & 7) == 0) wg = (wg & 7) * (nwg >> 3) + (wg >> 3);
+    const int per_group = group_m * tiles_n;
+    const int g = wg / per_group;
+    const int first_m = g * group_m;
+    int gm = tiles_m - first_m;
+    if (gm > group_m) gm = group_m;
+    const int r = wg - g * per_group;
+    tm = first_m + r % gm;
+    tn = r / gm;
+}
+
+// ROWS x BK tile of doubles: direct (k-contiguous source) or transposed
+// (row-contiguous source), stored to LDS as [row][k] with stride BK+2.
+template <int ROWS, int BK, int NT, bool TRANS>
+struct TileIO64 {
+    static constexpr int NV = ROWS * BK / 2 / NT;  // double2 per thread
+    static_assert(NV >= 1 && NV * 2 * NT == ROWS * BK, "tile must split evenly into double2 per thread");
+    static constexpr int LDK = BK + 2;
+
+    __device__ static __forceinline__ void load(d2v (&reg)[NV], const double* __restrict__ src, int64_t ld,
+                                                int r0, int k0, int rows_lim, int k_lim, bool full,
+                                                int tid) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int idx = q * NT + tid;
+            if (!TRANS) {
+                const int r = idx / (BK / 2), kq = (idx % (BK / 2)) * 2;
+                const int gr = r0 + r, gk = k0 + kq;
+                if (full) {
+                    reg[q] = *(const d2v*)(src + (int64_t)gr * ld + gk);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 2; ++e)
+                        reg[q][e] = (gr < rows_lim && gk + e < k_lim) ? src[(int64_t)gr * ld + gk + e] : 0.0;
+                }
+            } else {
+                const int k = idx % BK, rq = (idx / BK) * 2;
+                const int gk = k0 + k, gr = r0 + rq;
+                if (full) {
+                    reg[q] = *(const d2v*)(src + (int64_t)gk * ld + gr);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 2; ++e)
+                        reg[q][e] = (gk < k_lim && gr + e < rows_lim) ? src[(int64_t)gk * ld + gr + e] : 0.0;
+                }
+            }
+        }
+    }
+
+    __device__ static __forceinline__ void store(const d2v (&reg)[NV], double* lds, int tid) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int idx = q * NT + tid;
+            if (!TRANS) {
+                const int r = idx / (BK / 2), kq = (idx % (BK / 2)) * 2;
+                *(d2v*)(lds + r * LDK + kq) = reg[q];
+            } else {
+                const int k = idx % BK, rq = (idx / BK) * 2;
+                lds[(rq + 0) * LDK + k] = reg[q][0];
+                lds[(rq + 1) * LDK + k] = reg[q][1];
+            }
+        }
+    }
+};
+
+template <int WM, int WN, int TM, int TN, int BK, bool TA, bool TB>
+__global__ __launch_bounds__(WM* WN * 64) void dgemm_mfma_kernel(
+    const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C, int M, int N, int K,
+    int64_t lda, int64_t ldb, int64_t ldc, int64_t sA, int64_t sB, int64_t sC, double alpha, double beta,
+    int tiles_m, int tiles_n, int group_m, int vec_ok) {
+    constexpr int NT = WM * WN * 64;
+    constexpr int BM = WM * TM * 16, BN = WN * TN * 16;
+    constexpr int LDK = BK + 2;
+    static_assert(BK % 8 == 0, "BK must cover whole k-groups of 8");
+    typedef TileIO64<BM, BK, NT, TA> IoA;
+    typedef TileIO64<BN, BK, NT, !TB> IoB;
+
+    extern __shared__ double smemd[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+
+    A += (int64_t)blockIdx.y * sA;
+    B += (int64_t)blockIdx.y * sB;
+    C += (int64_t)blockIdx.y * sC;
+
+    int tmi, tni;
+    tile_coords_f64(blockIdx.x, gridDim.x, tiles_m, tiles_n, group_m, tmi, tni);
+    const int m0 = tmi * BM, n0 = tni * BN;
+    const bool full_mn = vec_ok && (m0 + BM <= M) && (n0 + BN <= N);
+
+    d4v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
+
+    d2v ra[IoA::NV], rb[IoB::NV];
+    const int KT = (K + BK - 1) / BK;
+
+    auto gload = [&](int t) {
+        const int k0 = t * BK;
+        const bool full = full_mn && (k0 + BK <= K);
+        IoA::load(ra, A, lda, m0, k0, M, K, full, tid);
+        IoB::load(rb, B, ldb, n0, k0, N, K, full, tid);
+    };
+
+    gload(0);
+    IoA::store(ra, smemd, tid);
+    IoB::store(rb, smemd + BM * LDK, tid);
+    __syncthreads();
+
+    const int arow = wm * TM * 16 + (lane & 15);
+    const int brow = wn * TN * 16 + (lane & 15);
+    const int kq = 2 * (lane >> 4);
+
+    for (int t = 0; t < KT; ++t) {
+        const double* As = smemd + (t & 1) * (BM + BN) * LDK;
+        const double* Bs = As + BM * LDK;
+        if (t + 1 < KT) gload(t + 1);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 8) {
+            d2v a[TM], b[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a[i] = *(const d2v*)(As + (arow + i * 16) * LDK + kk + kq);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) b[j] = *(const d2v*)(Bs + (brow + j * 16) * LDK + kk + kq);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+        }
+        if (t + 1 < KT) {
+            double* nA = smemd + ((t + 1) & 1) * (BM + BN) * LDK;
+            IoA::store(ra, nA, tid);
+            IoB::store(rb, nA + BM * LDK, tid);
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm * TM * 16 + i * 16 + (lane >> 4) * 4 + r;
+                const int col = n0 + wn * TN * 16 + j * 16 + (lane & 15);
+                if (row < M && col < N) {
+                    double* p = C + (int64_t)row * ldc + col;
+                    double v = alpha * acc[i][j][r];
+                    if (beta != 0.0) v += beta * *p;
+                    *p = v;
+                }
+            }
+}
+
+// Instantiated configurations: (WM, WN, TM, TN, BK).
+#define GAT_F64_MFMA_CFGS(X) \
+    X(2, 2, 2, 2, 16)        \
+    X(2, 2, 4, 4, 16)        \
+    X(2, 2, 4, 2, 16)        \
+    X(2, 2, 2, 2, 32)        \
+    X(2, 2, 4, 4, 8)         \
+    X(1, 1, 2, 2, 32)
+
+static const MfmaCfg kF64Cfgs[] = {
+#define X(wm, wn, tm, tn, bk) {wm, wn, tm, tn, bk, 16},
+    GAT_F64_MFMA_CFGS(X)
+#undef X
+};
+
+int mfma_f64_num_cfgs() { return (int)(sizeof(kF64Cfgs) / sizeof(kF64Cfgs[0])); }
+MfmaCfg mfma_f64_cfg(int i) { return kF64Cfgs[i]; }
+
+bool mfma_f64_has(const MfmaCfg& c) {
+    for (const MfmaCfg& k : kF64Cfgs)
+        if (k.waves_m == c.waves_m && k.waves_n == c.waves_n && k.tm == c.tm && k.tn == c.tn &&
+            k.bk == c.bk && k.blk == c.blk)
+            return true;
+    return false;
+}
+
+template <int WM, int WN, int TM, int TN, int BK>
+static int launch_one64(const GemmArgs& a, hipStream_t st) {
+    constexpr int BM = WM * TM * 16, BN = WN * TN * 16;
+    constexpr size_t smem = 2 * (size_t)(BM + BN) * (BK + 2) * sizeof(double);
+    const int tiles_m = (int)((a.M + BM - 1) / BM), tiles_n = (int)((a.N + BN - 1) / BN);
+    const double* A = (const double*)a.A;
+    const double* B = (const double*)a.B;
+    const int vec_ok = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && (a.lda % 2 == 0) &&
+                       (a.ldb % 2 == 0) && (a.strideA % 2 == 0) && (a.strideB % 2 == 0);
+    typedef void (*kfn)(const double*, const double*, double*, int, int, int, int64_t, int64_t, int64_t,
+                        int64_t, int64_t, int64_t, double, double, int, int, int, int);
+    kfn f;
+    if (!a.transA && !a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, false, false>;
+    else if (a.transA && !a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, true, false>;
+    else if (!a.transA && a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, false, true>;
+    else f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, true, true>;
+    if (smem > 65536) GAT_HIP_TRY(hipFuncSetAttribute((const void*)f,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    const int group_m = tiles_m < 8 ? tiles_m : 8;
+    dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)a.batch);
+    hipLaunchKernelGGL(f, grid, dim3(WM * WN * 64), smem, st, A, B, (double*)a.C, (int)a.M, (int)a.N,
+                       (int)a.K, a.lda, a.ldb, a.ldc, a.strideA, a.strideB, a.strideC, a.alpha, a.beta,
+                       tiles_m, tiles_n, group_m, vec_ok);
+    GAT_HIP_TRY(hipGetLastError());
+    return ST_OK;
+}
+
+int launch_gemm_mfma_f64(const MfmaCfg& c, const GemmArgs& a, hipStream_t st) {
+    if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return ST_OK;
+    if (a.M > INT32_MAX || a.N > INT32_MAX || a.K > INT32_MAX) return ST_INVALID;
+    if (c.blk != 16) {
+        set_error("mfma f64: only 16x16 blocks exist");
+        return ST_UNSUPPORTED;
+    }
+#define X(p_wm, p_wn, p_tm, p_tn, p_bk)                                                              \
+    if (c.waves_m == p_wm && c.waves_n == p_wn && c.tm == p_tm && c.tn == p_tn && c.bk == p_bk)      \
+        return launch_one64<p_wm, p_wn, p_tm, p_tn, p_bk>(a, st);
+    GAT_F64_MFMA_CFGS(X)
+#undef X
+    set_error("mfma f64: configuration not instantiated");
+    return ST_UNSUPPORTED;
+}
+
+}  // namespace gat

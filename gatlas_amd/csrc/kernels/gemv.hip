@@ -1,0 +1,288 @@
+// GEMV:  y = alpha*op(A)*x + beta*y   (FP32 / FP64), HBM-bound.
+//
+// Native, A row-major (op(A) = A, M x N, row stride lda):
+//   each wavefront owns ROWS consecutive rows and walks them together, every
+//   lane loading one 16-byte vector per row per step (1 KiB contiguous per
+//   row per wave-load, non-temporal since A is streamed once).  x is read
+//   once per step and reused across the ROWS rows.  The per-lane partial
+//   sums are combined by a wavefront butterfly reduction and lane 0 applies
+//   the alpha/beta epilogue.
+// Native, A transposed (A stored N x M row-major, op(A)[i][j] = A[j*lda + i]):
+//   outputs are contiguous in memory, so a work-item owns 4 (FP32) or 2
+//   (FP64) consecutive outputs and the j range is split into S slices across
+//   work-groups.  Slice partial sums go to a workspace and a second kernel
+//   adds the S partials in slice order (deterministic, no atomics) and
+//   applies the epilogue.
+// Compat (reference geometry, KernelMatvecBuffer.hpp:180-325): work-item g
+//   owns rows g*bh .. g*bh+bh-1, each a j-ascending chain of separately
+//   rounded multiply and add -- the arithmetic of the reference's host check.
+#include "../kernels.hpp"
+
+namespace gat {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename T, typename V, int ROWS, int UNROLL>
+__global__ __launch_bounds__(256) void gemv_rows_kernel(int64_t M, int64_t N, T alpha, const T* __restrict__ A,
+                                                         int64_t lda, const T* __restrict__ x, T beta,
+                                                         T* __restrict__ y, int vec_ok) {
+    constexpr int W = sizeof(V) / sizeof(T);
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t r0 = wave * ROWS;
+    if (r0 >= M) return;
+    T acc[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) acc[r] = 0;
+    int64_t c = 0;
+    if (vec_ok && r0 + ROWS <= M) {
+        const int64_t step = 64 * W;
+        const int64_t nfull = (N / (step * UNROLL)) * (step * UNROLL);
+        for (; c < nfull; c += step * UNROLL) {
+            V av[UNROLL][ROWS], xv[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                const int64_t cc = c + u * step + lane * W;
+                xv[u] = *(const V*)(x + cc);
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r)
+                    av[u][r] = __builtin_nontemporal_load((const V*)(A + (r0 + r) * lda + cc));
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+                    for (int e = 0; e < W; ++e) acc[r] += av[u][r][e] * xv[u][e];
+        }
+    }
+    // remainder (and unaligned / partial-row-block cases): scalar columns
+    for (int64_t cc = c + lane; cc < N; cc += 64) {
+        const T xv = x[cc];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+            if (r0 + r < M) acc[r] += A[(r0 + r) * lda + cc] * xv;
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) acc[r] = wave_sum<T>(acc[r]);
+    if (lane < ROWS && r0 + lane < M) {
+        T s = acc[0];
+#pragma unroll
+        for (int r = 1; r < ROWS; ++r)
+            if (lane == r) s = acc[r];
+        T v = alpha * s;
+        if (beta != (T)0) v += beta * y[r0 + lane];
+        y[r0 + lane] = v;
+    }
+}
+
+// Transposed: pass 1 partial sums.  Block = 256 work-items covering
+// 256*W consecutive outputs; blockIdx.y = slice.
+template <typename T, typename V>
+__global__ __launch_bounds__(256) void gemv_cols_partial_kernel(int64_t M, int64_t N, const T* __restrict__ A,
+                                                                 int64_t lda, const T* __restrict__ x,
+                                                                 T* __restrict__ part, int64_t jslice,
+                                                                 int vec_ok) {
+    constexpr int W = sizeof(V) / sizeof(T);
+    const int64_t i0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * W;
+    const int64_t j0 = (int64_t)blockIdx.y * jslice;
+    int64_t j1 = j0 + jslice;
+    if (j1 > N) j1 = N;
+    if (i0 >= M) return;
+    T* out = part + (int64_t)blockIdx.y * M;
+    if (vec_ok && i0 + W <= M) {
+        V acc = {};
+        int64_t j = j0;
+        for (; j + 4 <= j1; j += 4) {
+            V a0 = __builtin_nontemporal_load((const V*)(A + (j + 0) * lda + i0));
+            V a1 = __builtin_nontemporal_load((const V*)(A + (j + 1) * lda + i0));
+            V a2 = __builtin_nontemporal_load((const V*)(A + (j + 2) * lda + i0));
+            V a3 = __builtin_nontemporal_load((const V*)(A + (j + 3) * lda + i0));
+            acc += a0 * x[j + 0];
+            acc += a1 * x[j + 1];
+            acc += a2 * x[j + 2];
+            acc += a3 * x[j + 3];
+        }
+        for (; j < j1; ++j) acc += *(const V*)(A + j * lda + i0) * x[j];
+        *(V*)(out + i0) = acc;
+    } else {
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            if (i0 + e >= M) break;
+            T s = 0;
+            for (int64_t j = j0; j < j1; ++j) s += A[j * lda + i0 + e] * x[j];
+            out[i0 + e] = s;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_cols_reduce_kernel(int64_t M, int S, const T* __restrict__ part,
+                                                                T alpha, T beta, T* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= M) return;
+    T s = 0;
+    for (int k = 0; k < S; ++k) s += part[(int64_t)k * M + i];
+    T v = alpha * s;
+    if (beta != (T)0) v += beta * y[i];
+    y[i] = v;
+}
+
+static int gemv_cols_slices(size_t M, size_t N, int W) {
+    const size_t iblocks = (M + 256 * W - 1) / (256 * W);
+    size_t S = (2048 + iblocks - 1) / iblocks;
+    const size_t maxS = (N + 63) / 64;  // keep >= 64 rows of A per slice
+    if (S > maxS) S = maxS;
+    if (S < 1) S = 1;
+    if (S > 1024) S = 1024;
+    return (int)S;
+}
+
+size_t gemv_workspace_bytes(int dtype, int transA, size_t M, size_t N) {
+    if (!transA) return 0;
+    const int W = dtype == DT_F64 ? 2 : 4;
+    return (size_t)gemv_cols_slices(M, N, W) * M * (dtype == DT_F64 ? 8 : 4);
+}
+
+template <typename T, typename V>
+static int launch_native_t(int transA, size_t M, size_t N, T alpha, const T* A, size_t lda, const T* x,
+                           T beta, T* y, void* work, hipStream_t st) {
+    constexpr int W = sizeof(V) / sizeof(T);
+    if (!transA) {
+        constexpr int ROWS = 4;
+        const int vec_ok = ((uintptr_t)A % 16 == 0) && ((uintptr_t)x % 16 == 0) && (lda % W == 0);
+        const size_t waves = (M + ROWS - 1) / ROWS;
+        const size_t blocks = (waves + 3) / 4;
+        hipLaunchKernelGGL((gemv_rows_kernel<T, V, ROWS, 2>), dim3((unsigned)blocks), dim3(256), 0, st,
+                           (int64_t)M, (int64_t)N, alpha, A, (int64_t)lda, x, beta, y, vec_ok);
+        GAT_HIP_TRY(hipGetLastError());
+        return ST_OK;
+    }
+    if (!work) {
+        set_error("gemv transposed: workspace required");
+        return ST_INVALID;
+    }
+    const int S = gemv_cols_slices(M, N, W);
+    const int64_t jslice = (int64_t)((N + S - 1) / S);
+    // partial rows start at slice*M: vector stores need M % W == 0
+    const int vec_ok = ((uintptr_t)A % 16 == 0) && (lda % W == 0) && (M % W == 0) &&
+                       ((uintptr_t)work % 16 == 0);
+    const size_t iblocks = (M + 256 * W - 1) / (256 * W);
+    hipLaunchKernelGGL((gemv_cols_partial_kernel<T, V>), dim3((unsigned)iblocks, (unsigned)S), dim3(256), 0, st,
+                       (int64_t)M, (int64_t)N, A, (int64_t)lda, x, (T*)work, jslice, vec_ok);
+    GAT_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((gemv_cols_reduce_kernel<T>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st,
+                       (int64_t)M, S, (const T*)work, alpha, beta, y);
+    GAT_HIP_TRY(hipGetLastError());
+    return ST_OK;
+}
+
+int launch_gemv_native(int dtype, int transA, size_t M, size_t N, double alpha, const void* A, size_t lda,
+                       const void* x, double beta, void* y, void* work, hipStream_t st) {
+    if (M == 0) return ST_OK;
+    if (dtype == DT_F32)
+        return launch_native_t<float, f4v>(transA, M, N, (float)alpha, (const float*)A, lda, (const float*)x,
+                                           (float)beta, (float*)y, work, st);
+    if (dtype == DT_F64)
+        return launch_native_t<double, d2v>(transA, M, N, alpha, (const double*)A, lda, (const double*)x, beta,
+                                            (double*)y, work, st);
+    set_error("gemv: dtype must be f32 or f64");
+    return ST_INVALID;
+}
+
+// ---------------- reference geometry ----------------
+
+template <typename T>
+__device__ __forceinline__ T mad_rn(T acc, T a, T b);
+template <>
+__device__ __forceinline__ float mad_rn<float>(float acc, float a, float b) {
+    return __fadd_rn(acc, __fmul_rn(a, b));
+}
+template <>
+__device__ __forceinline__ double mad_rn<double>(double acc, double a, double b) {
+    return __dadd_rn(acc, __dmul_rn(a, b));
+}
+
+// Rows are processed CH at a time; each row's chain is j-ascending from 0,
+// so chunking changes only which rows share a pass over x, not any row's
+// arithmetic.
+template <typename T, int CH>
+__global__ void gemv_compat_kernel(size_t M, size_t N, size_t bh, int transA, int gemv, T alpha, T beta,
+                                   const T* __restrict__ A, const T* __restrict__ x, T* __restrict__ y) {
+    const size_t p = blockIdx.y;
+    A += p * M * N;
+    x += p * N;
+    y += p * M;
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t rbase = g * bh;
+    if (rbase >= M) return;
+    for (size_t i0 = 0; i0 < bh; i0 += CH) {
+        T acc[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] = 0;
+        for (size_t j = 0; j < N; ++j) {
+            const T xj = x[j];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const size_t r = rbase + i0 + c;
+                const T a = transA ? A[j * M + r] : A[r * N + j];
+                acc[c] = mad_rn<T>(acc[c], a, xj);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const size_t r = rbase + i0 + c;
+            if (gemv) {
+                T u, v;
+                if (sizeof(T) == 4) {
+                    u = __fmul_rn((float)alpha, (float)acc[c]);
+                    v = __fmul_rn((float)beta, (float)y[r]);
+                    y[r] = __fadd_rn((float)u, (float)v);
+                } else {
+                    u = __dmul_rn(alpha, acc[c]);
+                    v = __dmul_rn(beta, y[r]);
+                    y[r] = __dadd_rn(u, v);
+                }
+            } else {
+                y[r] = acc[c];
+            }
+        }
+    }
+}
+
+int launch_gemv_compat(int dtype, int vl, int transA, size_t M, size_t N, size_t gs, size_t bh, int packed,
+                       int gemv, double alpha, double beta, const void* A, const void* x, void* y,
+                       hipStream_t st) {
+    if (gs == 0 || gs > 1024 || bh == 0 || vl <= 0 || bh % vl != 0 || M % (gs * bh) != 0) {
+        set_error("gemv compat: invalid work-group or blocking");
+        return ST_INVALID;
+    }
+    const size_t groups = (M / bh) / gs;
+    const int ch = (bh % 8 == 0) ? 8 : (bh % 4 == 0) ? 4 : (bh % 2 == 0) ? 2 : 1;
+    dim3 grid((unsigned)groups, (unsigned)packed), block((unsigned)gs);
+#define GAT_GEMV_CH(T, C)                                                                              \
+    if (ch == C)                                                                                       \
+        hipLaunchKernelGGL((gemv_compat_kernel<T, C>), grid, block, 0, st, M, N, bh, transA, gemv, (T)alpha, \
+                           (T)beta, (const T*)A, (const T*)x, (T*)y);
+    if (dtype == DT_F32) {
+        GAT_GEMV_CH(float, 8) else GAT_GEMV_CH(float, 4) else GAT_GEMV_CH(float, 2) else GAT_GEMV_CH(float, 1)
+    } else if (dtype == DT_F64) {
+        GAT_GEMV_CH(double, 8) else GAT_GEMV_CH(double, 4) else GAT_GEMV_CH(double, 2) else GAT_GEMV_CH(double, 1)
+    } else {
+        set_error("gemv compat: dtype must be f32 or f64");
+        return ST_INVALID;
+    }
+#undef GAT_GEMV_CH
+    GAT_HIP_TRY(hipGetLastError());
+    return ST_OK;
+}
+
+}  // namespace gat
