@@ -157,12 +157,9 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
 
 MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
     const int64_t mn = a.M < a.N ? a.M : a.N;
-    if (dtype == DT_F64) {
-        if (mn >= 1024) return MfmaCfg{2, 2, 4, 4, 16, 16};
-        return MfmaCfg{2, 2, 2, 2, 16, 16};
-    }
+    if (dtype == DT_F64) return MfmaCfg{2, 2, 2, 2, 16, 16};
     // FP32 (and the FP32-output forms): large square-ish -> 128x128 tiles.
-    if (mn >= 1024) return MfmaCfg{2, 2, 2, 2, 16, 32};
+    if (mn >= 1024) return MfmaCfg{2, 2, 2, 2, 32, 32};
     if (mn >= 256) return MfmaCfg{2, 2, 1, 1, 32, 32};
     return MfmaCfg{1, 1, 2, 2, 32, 32};
 }

@@ -20,25 +20,21 @@
 
 namespace gat {
 
+// acc + a*b and alpha*acc + beta*c with every product rounded before the
+// add.  HIP compiles device code with fp-contract=fast; contraction is
+// switched off inside these helpers so no fused multiply-add is formed.
 template <typename T>
-__device__ __forceinline__ T cmad(T acc, T a, T b);
-template <>
-__device__ __forceinline__ float cmad<float>(float acc, float a, float b) {
-    return __fadd_rn(acc, __fmul_rn(a, b));
-}
-template <>
-__device__ __forceinline__ double cmad<double>(double acc, double a, double b) {
-    return __dadd_rn(acc, __dmul_rn(a, b));
+__device__ __forceinline__ T cmad(T acc, T a, T b) {
+#pragma clang fp contract(off)
+    const T p = a * b;
+    return acc + p;
 }
 template <typename T>
-__device__ __forceinline__ T cepi(T alpha, T acc, T beta, T c);
-template <>
-__device__ __forceinline__ float cepi<float>(float alpha, float acc, float beta, float c) {
-    return __fadd_rn(__fmul_rn(alpha, acc), __fmul_rn(beta, c));
-}
-template <>
-__device__ __forceinline__ double cepi<double>(double alpha, double acc, double beta, double c) {
-    return __dadd_rn(__dmul_rn(alpha, acc), __dmul_rn(beta, c));
+__device__ __forceinline__ T cepi(T alpha, T acc, T beta, T c) {
+#pragma clang fp contract(off)
+    const T u = alpha * acc;
+    const T v = beta * c;
+    return u + v;
 }
 
 template <typename T, int VL, int BH>

@@ -4,7 +4,8 @@
 // LDS with both operands stored k-contiguous ([m][k] and [n][k], row stride
 // BK+2 doubles) so that one ds_read_b128 hands a lane the operands of two
 // MFMAs.  MFMA s of a k-group takes, in lane l, k = kk + 2*(l/16) + s.
-// Output block layout: lane l holds rows 4*(l/16) + r, r < 4, of column l%16.
+// Output block layout (measured on gfx950; differs from the FP32 16x16x4
+// form): lane l, accumulator r holds row 4*r + l/16 of column l%16.
 #include "../kernels.hpp"
 
 namespace gat {
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(WM* WN * 64) void dgemm_mfma_kernel(
         for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * TM * 16 + i * 16 + (lane >> 4) * 4 + r;
+                const int row = m0 + wm * TM * 16 + i * 16 + r * 4 + (lane >> 4);
                 const int col = n0 + wn * TN * 16 + j * 16 + (lane & 15);
                 if (row < M && col < N) {
                     double* p = C + (int64_t)row * ldc + col;

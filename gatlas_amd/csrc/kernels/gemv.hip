@@ -200,15 +200,20 @@ int launch_gemv_native(int dtype, int transA, size_t M, size_t N, double alpha, 
 
 // ---------------- reference geometry ----------------
 
+// acc + a*b and alpha*acc + beta*c, each product rounded before the add
+// (contraction switched off: HIP device code defaults to fp-contract=fast).
 template <typename T>
-__device__ __forceinline__ T mad_rn(T acc, T a, T b);
-template <>
-__device__ __forceinline__ float mad_rn<float>(float acc, float a, float b) {
-    return __fadd_rn(acc, __fmul_rn(a, b));
+__device__ __forceinline__ T mad_rn(T acc, T a, T b) {
+#pragma clang fp contract(off)
+    const T p = a * b;
+    return acc + p;
 }
-template <>
-__device__ __forceinline__ double mad_rn<double>(double acc, double a, double b) {
-    return __dadd_rn(acc, __dmul_rn(a, b));
+template <typename T>
+__device__ __forceinline__ T epi_rn(T alpha, T acc, T beta, T c) {
+#pragma clang fp contract(off)
+    const T u = alpha * acc;
+    const T v = beta * c;
+    return u + v;
 }
 
 // Rows are processed CH at a time; each row's chain is j-ascending from 0,
@@ -241,16 +246,7 @@ __global__ void gemv_compat_kernel(size_t M, size_t N, size_t bh, int transA, in
         for (int c = 0; c < CH; ++c) {
             const size_t r = rbase + i0 + c;
             if (gemv) {
-                T u, v;
-                if (sizeof(T) == 4) {
-                    u = __fmul_rn((float)alpha, (float)acc[c]);
-                    v = __fmul_rn((float)beta, (float)y[r]);
-                    y[r] = __fadd_rn((float)u, (float)v);
-                } else {
-                    u = __dmul_rn(alpha, acc[c]);
-                    v = __dmul_rn(beta, y[r]);
-                    y[r] = __dadd_rn(u, v);
-                }
+                y[r] = epi_rn<T>(alpha, acc[c], beta, y[r]);
             } else {
                 y[r] = acc[c];
             }

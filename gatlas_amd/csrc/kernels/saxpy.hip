@@ -6,18 +6,15 @@
 
 namespace gat {
 
+// alpha*x + y with the product rounded before the add.  Contraction into a
+// fused multiply-add is switched off here: HIP compiles device code with
+// fp-contract=fast, which would otherwise fuse the two operations.
 template <typename T>
-__device__ __forceinline__ T mul_rn(T a, T b);
-template <>
-__device__ __forceinline__ float mul_rn<float>(float a, float b) { return __fmul_rn(a, b); }
-template <>
-__device__ __forceinline__ double mul_rn<double>(double a, double b) { return __dmul_rn(a, b); }
-template <typename T>
-__device__ __forceinline__ T add_rn(T a, T b);
-template <>
-__device__ __forceinline__ float add_rn<float>(float a, float b) { return __fadd_rn(a, b); }
-template <>
-__device__ __forceinline__ double add_rn<double>(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ T axpy1(T alpha, T x, T y) {
+#pragma clang fp contract(off)
+    const T p = alpha * x;
+    return p + y;
+}
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d2v __attribute__((ext_vector_type(2)));
@@ -46,7 +43,7 @@ __global__ __launch_bounds__(256) void axpy_stream_kernel(size_t nvec, T alpha, 
         if (i < nvec) {
             V r;
 #pragma unroll
-            for (int e = 0; e < W; ++e) r[e] = add_rn<T>(mul_rn<T>(alpha, xv[u][e]), yv[u][e]);
+            for (int e = 0; e < W; ++e) r[e] = axpy1<T>(alpha, xv[u][e], yv[u][e]);
             __builtin_nontemporal_store(r, z + i);
         }
     }
@@ -56,7 +53,7 @@ template <typename T>
 __global__ void axpy_tail_kernel(size_t start, size_t n, T alpha, const T* __restrict__ x,
                                  const T* __restrict__ y, T* __restrict__ z) {
     const size_t i = start + blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i < n) z[i] = add_rn<T>(mul_rn<T>(alpha, x[i]), y[i]);
+    if (i < n) z[i] = axpy1<T>(alpha, x[i], y[i]);
 }
 
 template <typename T, typename V>
@@ -114,7 +111,7 @@ __global__ void saxpy_compat_kernel(size_t M, size_t N, size_t gH, size_t gW, si
 #pragma unroll
             for (int e = 0; e < VL; ++e) {
                 const size_t s = v * VL + e;
-                Z[s] = add_rn<T>(mul_rn<T>(alpha, X[s]), Y[s]);
+                Z[s] = axpy1<T>(alpha, X[s], Y[s]);
             }
         }
     } else {
@@ -130,7 +127,7 @@ __global__ void saxpy_compat_kernel(size_t M, size_t N, size_t gH, size_t gW, si
 #pragma unroll
                 for (int e = 0; e < VL; ++e) {
                     const size_t s = v * VL + e;
-                    Z[s] = add_rn<T>(mul_rn<T>(alpha, X[s]), Y[s]);
+                    Z[s] = axpy1<T>(alpha, X[s], Y[s]);
                 }
             }
     }

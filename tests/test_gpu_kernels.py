@@ -38,12 +38,22 @@ def test_fill_uniform_matches_host_restatement(ctx):
 
 
 # ------------------------------------------------------------------ SAXPY
+# The reference's first posrand() draw is 11/2^48 (glibc's unseeded
+# drand48), which makes alpha*x vanish next to y; every bit-exact test
+# therefore also runs with a fixed alpha (and beta) of order one, where a
+# fused multiply-add or a wrong rounding shows up.
+STRONG_ALPHA, STRONG_BETA = 0.7109375, -1.3125
+
+
 @pytest.mark.parametrize("n", [1, 3, 4, 1023, 65536 + 5, 1 << 20])
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
-def test_axpy_native_bitexact(ctx, n, dt):
-    _say("axpy", n, dt.__name__)
+@pytest.mark.parametrize("mode", ["paranoid", "strong"])
+def test_axpy_native_bitexact(ctx, n, dt, mode):
+    _say("axpy", n, dt.__name__, mode)
     r = ora.Drand48(0)
     alpha = r.posrand(dt)
+    if mode == "strong":
+        alpha = dt(STRONG_ALPHA)
     x = r.fillrand(n, dt)
     y = r.fillrand(n, dt)
     dx, dy, dz = ctx.to_device(x), ctx.to_device(y), ctx.empty(n, dt)
@@ -69,7 +79,8 @@ def test_axpy_compat_bitexact(ctx, vl, M, N, gH, gW, bh, bw):
     _say("axpy_compat", vl, M, N, gH, gW, bh, bw)
     for dt in (np.float32, np.float64):
         r = ora.Drand48(0)
-        alpha = r.posrand(dt)
+        r.posrand(dt)
+        alpha = dt(STRONG_ALPHA)
         x = r.fillrand(M * N, dt)
         y = r.fillrand(M * N, dt)
         dx, dy, dz = ctx.to_device(x), ctx.to_device(y), ctx.empty(M * N, dt)
@@ -135,14 +146,17 @@ def test_gemv_test_pattern(ctx):
 
 @pytest.mark.parametrize("ta,gs,bh,vl,gemv", [(0, 64, 4, 4, 0), (1, 64, 8, 4, 1), (0, 16, 2, 2, 1),
                                               (1, 32, 1, 1, 0), (0, 8, 16, 4, 1)])
-def test_gemv_compat_bitexact(ctx, ta, gs, bh, vl, gemv):
+@pytest.mark.parametrize("mode", ["paranoid", "strong"])
+def test_gemv_compat_bitexact(ctx, ta, gs, bh, vl, gemv, mode):
     # The reference's paranoid setup: alpha, beta, A, x, y from posrand.
-    _say("gemv_compat", ta, gs, bh, vl, gemv)
+    _say("gemv_compat", ta, gs, bh, vl, gemv, mode)
     M, N = 512, 384
     for dt in (np.float32, np.float64):
         r = ora.Drand48(0)
         alpha = r.posrand(dt) if gemv else dt(1)
         beta = r.posrand(dt) if gemv else dt(1)
+        if gemv and mode == "strong":
+            alpha, beta = dt(STRONG_ALPHA), dt(STRONG_BETA)
         A = r.fillrand(M * N, dt)
         x = r.fillrand(N, dt)
         y0 = r.fillrand(M, dt) if gemv else np.zeros(M, dt)
@@ -159,13 +173,16 @@ def test_gemv_compat_bitexact(ctx, ta, gs, bh, vl, gemv):
     (0, 0, 16, 8, 4, 0), (1, 0, 8, 4, 4, 1), (0, 1, 10, 5, 2, 1), (1, 1, 16, 2, 2, 0),
     (0, 0, 8, 3, 1, 1), (0, 0, 12, 12, 4, 0),
 ])
-def test_gemm_compat_bitexact(ctx, ta, tb, gs, bh, vl, gemm):
-    _say("gemm_compat", ta, tb, gs, bh, vl, gemm)
+@pytest.mark.parametrize("mode", ["paranoid", "strong"])
+def test_gemm_compat_bitexact(ctx, ta, tb, gs, bh, vl, gemm, mode):
+    _say("gemm_compat", ta, tb, gs, bh, vl, gemm, mode)
     M, N, K = gs * bh * 2, gs * vl * 3, gs * vl * 4
     for dt in (np.float32, np.float64):
         r = ora.Drand48(0)
         alpha = r.posrand(dt) if gemm else dt(1)
         beta = r.posrand(dt) if gemm else dt(1)
+        if gemm and mode == "strong":
+            alpha, beta = dt(STRONG_ALPHA), dt(STRONG_BETA)
         A = r.fillrand(M * K, dt)
         B = r.fillrand(K * N, dt)
         C0 = r.fillrand(M * N, dt) if gemm else np.zeros(M * N, dt)
