@@ -65,6 +65,46 @@ SIGNATURES = {
     "gat_axpy": (_c_int, [_c_vp, _c_int, _c_sz, _c_dbl, _c_vp, _c_vp, _c_vp]),
     "gat_axpy_compat": (_c_int, [_c_vp, _c_int, _c_sz, _c_sz, _c_dbl, _c_vp, _c_vp, _c_vp, _c_sz, _c_sz,
                                  _c_sz, _c_sz, _c_sz, _c_sz]),
+    # parameter interface
+    "gat_kernel_new": (_c_vp, [_c_vp, _c_int, _c_int, _c_int, _c_int]),
+    "gat_kernel_free": (None, [_c_vp]),
+    "gat_kernel_name": (ctypes.c_char_p, [_c_vp]),
+    "gat_kernel_param_count": (_c_int, [_c_vp]),
+    "gat_kernel_set_params": (_c_int, [_c_vp, ctypes.POINTER(_c_sz), _c_sz]),
+    "gat_kernel_valid": (_c_int, [_c_vp]),
+    "gat_kernel_get_params": (_c_int, [_c_vp, ctypes.POINTER(_c_sz), _c_sz]),
+    "gat_kernel_set_paranoid": (_c_int, [_c_vp]),
+    "gat_kernel_set_packed": (_c_int, [_c_vp, _c_sz]),
+    "gat_kernel_total_variations": (_c_sz, [_c_vp]),
+    "gat_kernel_flops": (_c_sz, [_c_vp]),
+    "gat_kernel_extra_detail": (_c_int, [_c_vp, ctypes.POINTER(_c_sz), _c_sz]),
+    "gat_kernel_work_items": (_c_int, [_c_vp, ctypes.POINTER(_c_sz), ctypes.POINTER(_c_sz), ctypes.POINTER(_c_sz),
+                                       ctypes.POINTER(_c_sz), _c_sz]),
+    "gat_kernel_print": (ctypes.c_char_p, [_c_vp]),
+    "gat_kernel_build": (_c_int, [_c_vp]),
+    "gat_kernel_set_args": (_c_int, [_c_vp, _c_int]),
+    "gat_kernel_run": (_c_int, [_c_vp, _c_sz, ctypes.POINTER(_c_dbl)]),
+    "gat_kernel_sync_output": (_c_int, [_c_vp]),
+    "gat_kernel_check": (_c_int, [_c_vp, _c_int, ctypes.POINTER(_c_dbl)]),
+    # journal
+    "gat_journal_open": (_c_vp, [ctypes.c_char_p]),
+    "gat_journal_close": (None, [_c_vp]),
+    "gat_journal_load": (_c_int, [_c_vp]),
+    "gat_journal_purge": (_c_int, [_c_vp, _c_int]),
+    "gat_journal_good": (_c_sz, [_c_vp]),
+    "gat_journal_state": (_c_int, [_c_vp, _c_vp, ctypes.POINTER(_c_sz), _c_sz]),
+    "gat_journal_time": (_c_int, [_c_vp, _c_vp, ctypes.POINTER(_c_sz), _c_sz, _c_sz]),
+    "gat_journal_take": (_c_int, [_c_vp, _c_vp, ctypes.POINTER(_c_sz), _c_sz, _c_int]),
+    # ranking / search / tuning
+    "gat_rank_bench": (_c_int, [_c_sz, ctypes.POINTER(_c_int), ctypes.POINTER(_c_dbl), _c_sz]),
+    "gat_mark_bench": (_c_int, [_c_sz, ctypes.POINTER(_c_int), ctypes.POINTER(_c_sz), _c_sz]),
+    "gat_mark_bench_avg": (_c_int, [_c_sz, ctypes.POINTER(_c_int), ctypes.POINTER(_c_dbl), _c_sz]),
+    "gat_search_params": (_c_int, [_c_vp, _c_sz, _c_sz, _c_sz, _c_sz, _c_int, _c_sz, _c_sz, _c_sz, _c_int, _c_int,
+                                   ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.POINTER(_c_sz), _c_sz]),
+    "gat_tune_gemm": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, ctypes.c_char_p, _c_int,
+                               ctypes.POINTER(TileCfg), ctypes.POINTER(_c_dbl)]),
+    "gat_tuned_set": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, ctypes.POINTER(TileCfg)]),
+    "gat_tuned_get": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, ctypes.POINTER(TileCfg)]),
 }
 
 _lib = None

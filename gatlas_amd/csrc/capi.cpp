@@ -6,6 +6,7 @@
 #include <unordered_set>
 
 #include "../../include/gatlas.h"
+#include "capi_internal.hpp"
 #include "context.hpp"
 #include "kernels.hpp"
 
@@ -19,9 +20,6 @@ const char* last_error() { return g_err.c_str(); }
 
 using namespace gat;
 
-struct gat_ctx {
-    Context impl;
-};
 
 #define CTX_CHECK(ctx)                          \
     do {                                        \

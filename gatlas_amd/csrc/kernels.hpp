@@ -53,8 +53,14 @@ int launch_scale_columns(float* a, size_t rows, size_t cols, uint64_t seed, uint
 
 // ---- SAXPY (saxpy.hip) ----
 // z = alpha*x + y, n elements, separate multiply and add roundings.
+// Streaming launch shape: `wg` work-items per group, `unroll` 16-byte
+// vectors per operand per work-item, non-temporal or cached accesses.
+struct AxpyCfg {
+    int wg, unroll, nontemporal;
+};
+bool axpy_cfg_ok(const AxpyCfg& c);
 int launch_saxpy_native(int dtype, size_t n, double alpha, const void* x, const void* y, void* z,
-                        hipStream_t st);
+                        hipStream_t st, const AxpyCfg* cfg = nullptr);
 // Reference geometry (KernelSaxpyBuffer.hpp:143-225): 1-D when N == vl.
 int launch_saxpy_compat(int dtype, int vl, size_t M, size_t N, size_t gH, size_t gW, size_t bh,
                         size_t bw, int packed, double alpha, const void* x, const void* y, void* z,
@@ -64,9 +70,16 @@ int launch_saxpy_compat(int dtype, int vl, size_t M, size_t N, size_t gH, size_t
 // y = alpha*op(A)*x + beta*y.  Native: wavefront-cooperative rows (row-major)
 // or column-split partials with a deterministic second pass (transposed);
 // `work` must hold gemv_workspace_bytes() bytes.
+// Row-major launch shape: `wg` work-items per group, `rows` rows per
+// wavefront, `unroll` 16-byte loads in flight per row per lane.
+struct GemvCfg {
+    int wg, rows, unroll;
+};
+bool gemv_cfg_ok(const GemvCfg& c);
 size_t gemv_workspace_bytes(int dtype, int transA, size_t M, size_t N);
 int launch_gemv_native(int dtype, int transA, size_t M, size_t N, double alpha, const void* A,
-                       size_t lda, const void* x, double beta, void* y, void* work, hipStream_t st);
+                       size_t lda, const void* x, double beta, void* y, void* work, hipStream_t st,
+                       const GemvCfg* cfg = nullptr);
 // Reference geometry (KernelMatvecBuffer.hpp:180-325): each work-item owns bh
 // rows, j-ascending unfused chain per row.
 int launch_gemv_compat(int dtype, int vl, int transA, size_t M, size_t N, size_t gs, size_t bh,

@@ -31,7 +31,7 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 template <typename T, typename V, int ROWS, int UNROLL>
-__global__ __launch_bounds__(256) void gemv_rows_kernel(int64_t M, int64_t N, T alpha, const T* __restrict__ A,
+__global__ __launch_bounds__(512) void gemv_rows_kernel(int64_t M, int64_t N, T alpha, const T* __restrict__ A,
                                                          int64_t lda, const T* __restrict__ x, T beta,
                                                          T* __restrict__ y, int vec_ok) {
     constexpr int W = sizeof(V) / sizeof(T);
@@ -152,17 +152,38 @@ size_t gemv_workspace_bytes(int dtype, int transA, size_t M, size_t N) {
     return (size_t)gemv_cols_slices(M, N, W) * M * (dtype == DT_F64 ? 8 : 4);
 }
 
+template <typename T, typename V, int ROWS, int UNROLL>
+static void launch_rows(size_t M, size_t N, int wg, T alpha, const T* A, size_t lda, const T* x, T beta, T* y,
+                        int vec_ok, hipStream_t st) {
+    const size_t waves = (M + ROWS - 1) / ROWS;
+    const size_t wpb = (size_t)wg / 64;
+    const size_t blocks = (waves + wpb - 1) / wpb;
+    hipLaunchKernelGGL((gemv_rows_kernel<T, V, ROWS, UNROLL>), dim3((unsigned)blocks), dim3((unsigned)wg), 0, st,
+                       (int64_t)M, (int64_t)N, alpha, A, (int64_t)lda, x, beta, y, vec_ok);
+}
+
+bool gemv_cfg_ok(const GemvCfg& c) {
+    return (c.wg == 64 || c.wg == 128 || c.wg == 256 || c.wg == 512) &&
+           (c.rows == 1 || c.rows == 2 || c.rows == 4 || c.rows == 8) &&
+           (c.unroll == 1 || c.unroll == 2 || c.unroll == 4);
+}
+
 template <typename T, typename V>
-static int launch_native_t(int transA, size_t M, size_t N, T alpha, const T* A, size_t lda, const T* x,
-                           T beta, T* y, void* work, hipStream_t st) {
+static int launch_native_t(int transA, const GemvCfg& cfg, size_t M, size_t N, T alpha, const T* A, size_t lda,
+                           const T* x, T beta, T* y, void* work, hipStream_t st) {
     constexpr int W = sizeof(V) / sizeof(T);
+    if (!gemv_cfg_ok(cfg)) {
+        set_error("gemv: unsupported (work-group, rows, unroll)");
+        return ST_UNSUPPORTED;
+    }
     if (!transA) {
-        constexpr int ROWS = 4;
         const int vec_ok = ((uintptr_t)A % 16 == 0) && ((uintptr_t)x % 16 == 0) && (lda % W == 0);
-        const size_t waves = (M + ROWS - 1) / ROWS;
-        const size_t blocks = (waves + 3) / 4;
-        hipLaunchKernelGGL((gemv_rows_kernel<T, V, ROWS, 2>), dim3((unsigned)blocks), dim3(256), 0, st,
-                           (int64_t)M, (int64_t)N, alpha, A, (int64_t)lda, x, beta, y, vec_ok);
+#define GAT_ROWS(R, U)                                                                              \
+    if (cfg.rows == R && cfg.unroll == U)                                                           \
+        launch_rows<T, V, R, U>(M, N, cfg.wg, alpha, A, lda, x, beta, y, vec_ok, st);
+        GAT_ROWS(1, 1) GAT_ROWS(1, 2) GAT_ROWS(1, 4) GAT_ROWS(2, 1) GAT_ROWS(2, 2) GAT_ROWS(2, 4)
+        GAT_ROWS(4, 1) GAT_ROWS(4, 2) GAT_ROWS(4, 4) GAT_ROWS(8, 1) GAT_ROWS(8, 2) GAT_ROWS(8, 4)
+#undef GAT_ROWS
         GAT_HIP_TRY(hipGetLastError());
         return ST_OK;
     }
@@ -186,13 +207,14 @@ static int launch_native_t(int transA, size_t M, size_t N, T alpha, const T* A, 
 }
 
 int launch_gemv_native(int dtype, int transA, size_t M, size_t N, double alpha, const void* A, size_t lda,
-                       const void* x, double beta, void* y, void* work, hipStream_t st) {
+                       const void* x, double beta, void* y, void* work, hipStream_t st, const GemvCfg* cfg) {
     if (M == 0) return ST_OK;
+    const GemvCfg c = cfg ? *cfg : GemvCfg{256, 4, 2};
     if (dtype == DT_F32)
-        return launch_native_t<float, f4v>(transA, M, N, (float)alpha, (const float*)A, lda, (const float*)x,
+        return launch_native_t<float, f4v>(transA, c, M, N, (float)alpha, (const float*)A, lda, (const float*)x,
                                            (float)beta, (float*)y, work, st);
     if (dtype == DT_F64)
-        return launch_native_t<double, d2v>(transA, M, N, alpha, (const double*)A, lda, (const double*)x, beta,
+        return launch_native_t<double, d2v>(transA, c, M, N, alpha, (const double*)A, lda, (const double*)x, beta,
                                             (double*)y, work, st);
     set_error("gemv: dtype must be f32 or f64");
     return ST_INVALID;

@@ -19,32 +19,41 @@ __device__ __forceinline__ T axpy1(T alpha, T x, T y) {
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d2v __attribute__((ext_vector_type(2)));
 
-// Native streaming kernel: each thread moves UNROLL 16-byte vectors per
-// operand, strided by the block size so that every load instruction of a
+// Native streaming kernel: each work-item moves UNROLL 16-byte vectors per
+// operand, strided by the group size so that every load instruction of a
 // wavefront covers 1 KiB of contiguous HBM; all loads are issued before any
-// store, and the streams bypass cache residency (non-temporal).
-template <typename T, typename V, int UNROLL>
-__global__ __launch_bounds__(256) void axpy_stream_kernel(size_t nvec, T alpha, const V* __restrict__ x,
-                                                           const V* __restrict__ y, V* __restrict__ z) {
+// store.  NT selects non-temporal accesses (the streams are touched once).
+template <typename T, typename V, int UNROLL, bool NT>
+__global__ __launch_bounds__(1024) void axpy_stream_kernel(size_t nvec, T alpha, const V* __restrict__ x,
+                                                            const V* __restrict__ y, V* __restrict__ z) {
     constexpr int W = sizeof(V) / sizeof(T);
-    const size_t base = (size_t)blockIdx.x * 256 * UNROLL + threadIdx.x;
+    const size_t bs = blockDim.x;
+    const size_t base = (size_t)blockIdx.x * bs * UNROLL + threadIdx.x;
     V xv[UNROLL], yv[UNROLL];
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
-        const size_t i = base + (size_t)u * 256;
+        const size_t i = base + (size_t)u * bs;
         if (i < nvec) {
-            xv[u] = __builtin_nontemporal_load(x + i);
-            yv[u] = __builtin_nontemporal_load(y + i);
+            if (NT) {
+                xv[u] = __builtin_nontemporal_load(x + i);
+                yv[u] = __builtin_nontemporal_load(y + i);
+            } else {
+                xv[u] = x[i];
+                yv[u] = y[i];
+            }
         }
     }
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
-        const size_t i = base + (size_t)u * 256;
+        const size_t i = base + (size_t)u * bs;
         if (i < nvec) {
             V r;
 #pragma unroll
             for (int e = 0; e < W; ++e) r[e] = axpy1<T>(alpha, xv[u][e], yv[u][e]);
-            __builtin_nontemporal_store(r, z + i);
+            if (NT)
+                __builtin_nontemporal_store(r, z + i);
+            else
+                z[i] = r;
         }
     }
 }
@@ -56,16 +65,35 @@ __global__ void axpy_tail_kernel(size_t start, size_t n, T alpha, const T* __res
     if (i < n) z[i] = axpy1<T>(alpha, x[i], y[i]);
 }
 
+bool axpy_cfg_ok(const AxpyCfg& c) {
+    return c.wg >= 64 && c.wg <= 1024 && c.wg % 64 == 0 &&
+           (c.unroll == 1 || c.unroll == 2 || c.unroll == 4 || c.unroll == 8) &&
+           (c.nontemporal == 0 || c.nontemporal == 1);
+}
+
+template <typename T, typename V, int U, bool NT>
+static void launch_stream(size_t nvec, int wg, T alpha, const T* x, const T* y, T* z, hipStream_t st) {
+    const size_t per = (size_t)wg * U;
+    const size_t blocks = (nvec + per - 1) / per;
+    hipLaunchKernelGGL((axpy_stream_kernel<T, V, U, NT>), dim3((unsigned)blocks), dim3((unsigned)wg), 0, st, nvec,
+                       alpha, (const V*)x, (const V*)y, (V*)z);
+}
+
 template <typename T, typename V>
-static int launch_native_t(size_t n, T alpha, const T* x, const T* y, T* z, hipStream_t st) {
+static int launch_native_t(const AxpyCfg& c, size_t n, T alpha, const T* x, const T* y, T* z, hipStream_t st) {
     constexpr int W = sizeof(V) / sizeof(T);
-    constexpr int UNROLL = 4;
+    if (!axpy_cfg_ok(c)) {
+        set_error("saxpy: unsupported (work-group, unroll, non-temporal)");
+        return ST_UNSUPPORTED;
+    }
     const bool aligned = ((uintptr_t)x % 16 == 0) && ((uintptr_t)y % 16 == 0) && ((uintptr_t)z % 16 == 0);
     size_t nvec = aligned ? n / W : 0;
     if (nvec) {
-        const size_t blocks = (nvec + 256 * UNROLL - 1) / (256 * UNROLL);
-        hipLaunchKernelGGL((axpy_stream_kernel<T, V, UNROLL>), dim3((unsigned)blocks), dim3(256), 0, st, nvec,
-                           alpha, (const V*)x, (const V*)y, (V*)z);
+#define GAT_AXPY(U, NTV) \
+    if (c.unroll == U && c.nontemporal == NTV) launch_stream<T, V, U, (NTV != 0)>(nvec, c.wg, alpha, x, y, z, st);
+        GAT_AXPY(1, 0) GAT_AXPY(2, 0) GAT_AXPY(4, 0) GAT_AXPY(8, 0)
+        GAT_AXPY(1, 1) GAT_AXPY(2, 1) GAT_AXPY(4, 1) GAT_AXPY(8, 1)
+#undef GAT_AXPY
         GAT_HIP_TRY(hipGetLastError());
     }
     const size_t start = nvec * W;
@@ -79,12 +107,13 @@ static int launch_native_t(size_t n, T alpha, const T* x, const T* y, T* z, hipS
 }
 
 int launch_saxpy_native(int dtype, size_t n, double alpha, const void* x, const void* y, void* z,
-                        hipStream_t st) {
+                        hipStream_t st, const AxpyCfg* cfg) {
     if (n == 0) return ST_OK;
+    const AxpyCfg c = cfg ? *cfg : AxpyCfg{256, 4, 1};
     if (dtype == DT_F32)
-        return launch_native_t<float, f4v>(n, (float)alpha, (const float*)x, (const float*)y, (float*)z, st);
+        return launch_native_t<float, f4v>(c, n, (float)alpha, (const float*)x, (const float*)y, (float*)z, st);
     if (dtype == DT_F64)
-        return launch_native_t<double, d2v>(n, alpha, (const double*)x, (const double*)y, (double*)z, st);
+        return launch_native_t<double, d2v>(c, n, alpha, (const double*)x, (const double*)y, (double*)z, st);
     set_error("saxpy: dtype must be f32 or f64");
     return ST_INVALID;
 }

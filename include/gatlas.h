@@ -157,6 +157,99 @@ int gat_axpy(gat_ctx *ctx, int dtype, size_t n, double alpha, const void *x, con
 int gat_axpy_compat(gat_ctx *ctx, int dtype, size_t M, size_t N, double alpha, const void *x, const void *y,
                     void *z, size_t gH, size_t gW, size_t bh, size_t bw, size_t vl, size_t packed);
 
+/* ---------------------------------------------------------------------- */
+/* Parameter interface: the KernelInterface boundary                      */
+/* (GatlasBenchmark.hpp:28-58) with the KernelBase{Matmul,Matvec,Saxpy}    */
+/* parameter vectors.  Parameter functions need no device (ctx may be     */
+/* NULL); build/set_args/run/check need a context.                        */
+/* ---------------------------------------------------------------------- */
+
+/* family GAT_MATMUL/MATVEC/SAXPY, dtype GAT_F32/F64 (MFMA also BF16, BF16X3,
+ * FP8), vl 1/2/4, variant GAT_BUFFER/IMAGE/MFMA/NATIVE.  Replaces the
+ * benchMatmul.cpp:460-485 kernel-type table.  NULL if no such kernel. */
+gat_kernel *gat_kernel_new(gat_ctx *ctx, int family, int dtype, int vl, int variant);
+void gat_kernel_free(gat_kernel *k);
+/* KernelInterface::kernelName, e.g. "matmulbufferfloat4" (KernelMatmulBuffer.hpp:58-62). */
+const char *gat_kernel_name(gat_kernel *k);
+/* 12 for matmul, 9 for matvec and saxpy (KernelBase*::getParams). */
+int gat_kernel_param_count(gat_kernel *k);
+/* KernelBase*::setParams (KernelBaseMatmul.cpp:327-362). */
+int gat_kernel_set_params(gat_kernel *k, const size_t *params, size_t n);
+/* KernelBase*::validParams (KernelBaseMatmul.cpp:263-291): 1 valid, 0 not. */
+int gat_kernel_valid(gat_kernel *k);
+/* KernelBase*::getParams (KernelBaseMatmul.cpp:293-325): count written, or <0 if invalid. */
+int gat_kernel_get_params(gat_kernel *k, size_t *out, size_t n);
+/* KernelInterface::paranoidCheck. */
+int gat_kernel_set_paranoid(gat_kernel *k);
+/* setPackedCalc (KernelBaseMatmul.hpp:235-248). */
+int gat_kernel_set_packed(gat_kernel *k, size_t packed);
+/* MatmulExtraParameter::totalVariations (KernelBaseMatmul.cpp:200). */
+size_t gat_kernel_total_variations(gat_kernel *k);
+/* KernelBase*::numberFlops (KernelBaseMatmul.cpp:378-383). */
+size_t gat_kernel_flops(gat_kernel *k);
+/* KernelInterface::extraParamDetail: number of digits, digits into out. */
+int gat_kernel_extra_detail(gat_kernel *k, size_t *out, size_t n);
+/* globalWorkItems / localWorkItems (KernelBaseMatmul.cpp:364-376): up to cap
+ * entries each, with their counts in *nglobal / *nlocal. */
+int gat_kernel_work_items(gat_kernel *k, size_t *global, size_t *nglobal, size_t *local, size_t *nlocal,
+                          size_t cap);
+/* KernelInterface::print: describes the compiled HIP instantiation. */
+const char *gat_kernel_print(gat_kernel *k);
+/* Bench::rebuildProgram (GatlasBenchmark.cpp:160-175): resolves the instantiation. */
+int gat_kernel_build(gat_kernel *k);
+/* KernelInterface::setArgs (KernelMatmulBuffer.hpp:83-171). */
+int gat_kernel_set_args(gat_kernel *k, int sync_input);
+/* Bench::run(ntrials, ...) without a journal (GatlasBenchmark.cpp:178-289): usec, 0 on failure. */
+int gat_kernel_run(gat_kernel *k, size_t ntrials, double *usec);
+/* KernelInterface::syncOutput. */
+int gat_kernel_sync_output(gat_kernel *k);
+/* KernelInterface::checkOutput: 1 pass, 0 fail; err = absdiff or componentwise error. */
+int gat_kernel_check(gat_kernel *k, int print, double *err);
+
+/* ---------------------------------------------------------------------- */
+/* Journal (GatlasBenchmark.hpp:62-95), byte-compatible text format:      */
+/*   "<kernelName>_<p0>_..._<pn>_\t<value>\n"                              */
+/* ---------------------------------------------------------------------- */
+typedef struct gat_journal gat_journal;
+gat_journal *gat_journal_open(const char *path);
+void gat_journal_close(gat_journal *j);
+/* Journal::loadMemo (GatlasBenchmark.cpp:56-73). */
+int gat_journal_load(gat_journal *j);
+/* Journal::purgeMemo (GatlasBenchmark.cpp:75-105): bad-key count, -1 on error. */
+int gat_journal_purge(gat_journal *j, int delete_times);
+/* Journal::memoGood. */
+size_t gat_journal_good(gat_journal *j);
+/* Journal::memoRunState (GatlasBenchmark.cpp:109-115): 0 missing, -1..-4 states. */
+int gat_journal_state(gat_journal *j, gat_kernel *k, const size_t *params, size_t n);
+/* Journal::memoTime (GatlasBenchmark.cpp:117-127): usec of a trial, -1 if absent. */
+int gat_journal_time(gat_journal *j, gat_kernel *k, const size_t *params, size_t n, size_t trial);
+/* Journal::takeMemo (GatlasBenchmark.cpp:129-137). */
+int gat_journal_take(gat_journal *j, gat_kernel *k, const size_t *params, size_t n, int value);
+
+/* AppUtil::rankBench / markBench (GatlasAppUtil.cpp:285-364); ok[] is 0/1. */
+int gat_rank_bench(size_t nth, const int *ok, const double *avg, size_t n);
+int gat_mark_bench(size_t topN, int *ok, const size_t *time, size_t n);
+int gat_mark_bench_avg(size_t topN, int *ok, const double *avg, size_t n);
+
+/* Candidate enumeration getParams (benchMatmul.cpp:217-304, benchMatvec.cpp:202-289,
+ * benchSaxpy.cpp:158-275); -1 marks a free parameter.  Writes up to max_out/count
+ * vectors, returns the total number of candidates.  For SAXPY, transB carries the
+ * fixed group width and K the fixed block width. */
+int gat_search_params(gat_kernel *k, size_t max_workgroup, size_t vl, size_t max_block_height,
+                      size_t max_group_size, int generalized, size_t M, size_t N, size_t K, int transA, int transB,
+                      long group_size, long block_height, long extra_param, size_t *out, size_t max_out);
+
+/* Autotuner (no reference counterpart; replaces the exhaustive OpenCL search for
+ * the matrix-core kernels): EM search over the instantiated tile table for one
+ * shape, journaled to journal_path, result installed for the shape's size bucket. */
+int gat_tune_gemm(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, size_t N, size_t K,
+                  const char *journal_path, int verbose, gat_tile_cfg *best, double *gflops);
+/* Install / query the tile choice of a size bucket (get returns 1 if tuned, 0 default). */
+int gat_tuned_set(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, size_t N, size_t K,
+                  const gat_tile_cfg *cfg);
+int gat_tuned_get(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, size_t N, size_t K,
+                  gat_tile_cfg *out);
+
 #ifdef __cplusplus
 }
 #endif
