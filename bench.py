@@ -32,6 +32,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-secondary", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-serve", action="store_true", help="skip the serving-loop figures")
+    p.add_argument("--no-grouped", action="store_true", help="skip the grouped-GEMM figures")
+    p.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of arrivals per serving run")
     return p.parse_args()
 
 
@@ -73,6 +76,95 @@ def cpu_baseline(n, budget_s):
             "sample": "%d of %d output rows of the %d^3 FP32 general-form product (%.1f s); reference "
                       "buffer-kernel arithmetic: k-ascending unfused multiply-add chains, OpenMP over rows"
                       % (rows2, n, n, t2)}
+
+
+def _pcts(v):
+    import numpy as np
+    v = np.sort(np.asarray(v))
+    return float(np.percentile(v, 50)), float(np.percentile(v, 99))
+
+
+def grouped_bench(ctx, dtype, reps=10):
+    """512 independent problems, M, N, K log-uniform in [64, 4096] (multiples
+    of 16), random transposes: one tuned, graph-captured persistent launch
+    against the per-problem launch loop (same tuned tiles)."""
+    import numpy as np
+
+    from gatlas_amd import _capi
+    rng = np.random.default_rng(2024)
+    v = np.clip(np.round(np.exp(rng.uniform(np.log(64), np.log(4096), (512, 3))) / 16) * 16, 64, 4096).astype(int)
+    t = rng.integers(0, 2, (512, 2))
+    probs, arrays = [], []
+    for i in range(512):
+        M, N, K = (int(x) for x in v[i])
+        ta, tb = int(t[i, 0]), int(t[i, 1])
+        if dtype == _capi.GAT_BF16:
+            f = ctx.empty(max(M * K, K * N))
+            dA, dB = ctx.empty(M * K, np.uint16), ctx.empty(K * N, np.uint16)
+            ctx.fill_uniform(f, 21, 2 * i)
+            ctx.convert_bf16_n(f, dA, M * K)
+            ctx.fill_uniform(f, 21, 2 * i + 1)
+            ctx.convert_bf16_n(f, dB, K * N)
+            f.free()
+        else:
+            dA, dB = ctx.empty(M * K), ctx.empty(K * N)
+            ctx.fill_uniform(dA, 21, 2 * i)
+            ctx.fill_uniform(dB, 21, 2 * i + 1)
+        dC = ctx.empty(M * N)
+        arrays += [dA, dB, dC]
+        probs.append(dict(transA=ta, transB=tb, M=M, N=N, K=K, alpha=1.0, beta=0.0, A=dA, B=dB, C=dC))
+    flops = sum(2.0 * p["M"] * p["N"] * p["K"] for p in probs)
+    ctx.sync()
+    t0 = time.perf_counter()
+    nb = ctx.grouped_tune(probs, dtype)
+    tune_s = time.perf_counter() - t0
+    g = ctx.grouped(probs, dtype, use_graph=True)
+    g.run()
+    ctx.sync()
+    gl = []
+    for _ in range(reps):
+        ctx.timer_start()
+        g.run()
+        gl.append(ctx.timer_stop())
+    loop = []
+
+    def per_problem():
+        for p in probs:
+            ctx.gemm(p["A"], p["B"], p["C"], p["M"], p["N"], p["K"], bool(p["transA"]), bool(p["transB"]),
+                     1.0, 0.0, dtype=dtype)
+    per_problem()
+    ctx.sync()
+    for _ in range(reps):
+        ctx.timer_start()
+        per_problem()
+        loop.append(ctx.timer_stop())
+    info = g.info()
+    g.close()
+    for a in arrays:
+        a.free()
+    g50, g99 = _pcts(gl)
+    l50, l99 = _pcts(loop)
+    return {"problems": 512, "gflop": round(flops / 1e9, 2), "tiles": info["tiles"], "grid": info["grid"],
+            "buckets_tuned": nb, "tune_s": round(tune_s, 2),
+            "grouped_TFLOP/s": round(flops / (g50 * 1e-3) / 1e12, 2), "grouped_ms_p50": round(g50, 3),
+            "grouped_ms_p99": round(g99, 3),
+            "loop_TFLOP/s": round(flops / (l50 * 1e-3) / 1e12, 2), "loop_ms_p50": round(l50, 3),
+            "loop_ms_p99": round(l99, 3), "speedup": round(l50 / g50, 3)}
+
+
+def serve_bench(ctx, rate, mode, seconds):
+    """Open-loop Poisson trace: 70% GEMM (grouped-workload shapes, FP32), 30%
+    GEMV (M, N log-uniform in [1024, 32768]), resident operands, 64 GB
+    admission budget."""
+    from gatlas_amd.device import Server
+    srv = Server(ctx, rate=float(rate), duration=float(seconds), mode=mode, seed=rate)
+    st = srv.run()
+    srv.close()
+    keys = ("requests", "launches", "wall_s", "tflops", "req_per_s", "p50_ms", "p99_ms", "p50_gemm_ms",
+            "p99_gemm_ms", "p50_gemv_ms", "p99_gemv_ms", "mean_queue_ms", "mean_batch")
+    out = {k: (round(st[k], 4) if isinstance(st[k], float) else st[k]) for k in keys}
+    out["peak_footprint_GB"] = round(st["peak_footprint"] / 1e9, 2)
+    return out
 
 
 def main():
@@ -120,10 +212,37 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
                      "frac": round(tflops / PEAK_FP32, 4), "traffic": None},
     }
+    sec = {}
+    if not args.no_secondary:
+        # mixed precision on the same operands (FP32 inputs resident in HBM)
+        ms_x3, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n, dtype=_capi.GAT_BF16X3), 10, 2)
+        tf_x3 = flops / (ms_x3 * 1e-3) / 1e12
+        sec["bf16x3_8192"] = {"ms": round(ms_x3, 4), "TFLOP/s": round(tf_x3, 2),
+                              "vs_fp32_path": round(tf_x3 / tflops, 3),
+                              "note": "FP32 inputs split to bf16 hi/lo in the kernel; 3 MFMA products; "
+                                      "TFLOP/s counted as 2MNK"}
+        a16, b16 = ctx.empty(n * n, np.uint16), ctx.empty(n * n, np.uint16)
+        ctx.convert_bf16(A, a16)
+        ctx.convert_bf16(B, b16)
+        ms_b, _ = timed(ctx, lambda: ctx.gemm(a16, b16, C, n, n, n, dtype=_capi.GAT_BF16), 10, 2)
+        tf_b = flops / (ms_b * 1e-3) / 1e12
+        sec["bf16_8192"] = {"ms": round(ms_b, 4), "TFLOP/s": round(tf_b, 2), "frac_bf16_peak": round(tf_b / 2500, 4)}
+        a16.free()
+        b16.free()
+        nb = (n // 128) * (n // 128)
+        A8, B8 = ctx.empty(n * n, np.uint8), ctx.empty(n * n, np.uint8)
+        sA, sB = ctx.empty(nb, np.float32), ctx.empty(nb, np.float32)
+        ms_q, _ = timed(ctx, lambda: (ctx.quantize_fp8(A, n, n, n, False, A8, sA),
+                                      ctx.quantize_fp8(B, n, n, n, True, B8, sB)), 3, 1)
+        ms_f8, _ = timed(ctx, lambda: ctx.gemm_fp8(A8, sA, B8, sB, C, n, n, n), 10, 2)
+        tf_f8 = flops / (ms_f8 * 1e-3) / 1e12
+        sec["fp8_8192"] = {"ms": round(ms_f8, 4), "TFLOP/s": round(tf_f8, 2), "frac_fp8_peak": round(tf_f8 / 5000, 4),
+                           "quantize_ms_both_operands": round(ms_q, 4)}
+        for a in (A8, B8, sA, sB):
+            a.free()
     for a in (A, B, C):
         a.free()
 
-    sec = {}
     if not args.no_secondary:
         # SGEMV y = A x + y, M = N = 16384
         m = 16384
@@ -158,6 +277,13 @@ def main():
         sec["dgemm_8192"] = {"ms": round(ms_d, 3), "TFLOP/s": round(tf, 2), "frac_fp64": round(tf / PEAK_FP64, 4)}
         for a in (Ad, Bd, Cd):
             a.free()
+    if not args.no_grouped:
+        for dt, name in ((_capi.GAT_F32, "f32"), (_capi.GAT_BF16, "bf16")):
+            sec["grouped512_" + name] = grouped_bench(ctx, dt)
+    if not args.no_serve:
+        for rate in (2000, 10000, 50000):
+            for mode, mname in ((_capi.GAT_SERVE_BATCHED, "batched"), (_capi.GAT_SERVE_LOOP, "loop")):
+                sec["serve_%dk_%s" % (rate // 1000, mname)] = serve_bench(ctx, rate, mode, args.serve_seconds)
     out["secondary"] = sec
     ctx.close()
 

@@ -28,8 +28,61 @@ class TileCfg(ctypes.Structure):
         return (self.waves_m, self.waves_n, self.tm, self.tn, self.bk, self.blk)
 
 
+class GemmDesc(ctypes.Structure):
+    _fields_ = [("transA", _c_int), ("transB", _c_int), ("M", _c_sz), ("N", _c_sz), ("K", _c_sz),
+                ("alpha", _c_dbl), ("A", _c_vp), ("lda", _c_sz), ("B", _c_vp), ("ldb", _c_sz),
+                ("beta", _c_dbl), ("C", _c_vp), ("ldc", _c_sz)]
+
+
+class GemvDesc(ctypes.Structure):
+    _fields_ = [("M", _c_sz), ("N", _c_sz), ("alpha", _c_dbl), ("A", _c_vp), ("lda", _c_sz), ("x", _c_vp),
+                ("beta", _c_dbl), ("y", _c_vp)]
+
+
+GAT_SERVE_BATCHED = 0
+GAT_SERVE_LOOP = 1
+
+
+class ServeCfg(ctypes.Structure):
+    _fields_ = [("rate", _c_dbl), ("duration", _c_dbl), ("gemm_fraction", _c_dbl), ("hbm_budget", _c_sz),
+                ("mode", _c_int), ("streams", _c_int), ("max_batch", _c_int), ("max_inflight", _c_int),
+                ("gemm_min", _c_int), ("gemm_max", _c_int), ("gemv_min", _c_int), ("gemv_max", _c_int),
+                ("keep", _c_int), ("keep_max", _c_int), ("seed", ctypes.c_ulonglong)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [(n, _c_sz) for n in ("requests", "gemm_requests", "gemv_requests", "launches", "kept")] + \
+               [(n, _c_dbl) for n in ("wall_s", "flops", "bytes", "tflops", "req_per_s", "p50_ms", "p99_ms",
+                                      "max_ms", "p50_gemm_ms", "p99_gemm_ms", "p50_gemv_ms", "p99_gemv_ms",
+                                      "mean_queue_ms", "mean_batch", "peak_footprint")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class ServeReq(ctypes.Structure):
+    _fields_ = [("kind", _c_int), ("transA", _c_int), ("transB", _c_int), ("shape", _c_int), ("M", _c_sz),
+                ("N", _c_sz), ("K", _c_sz), ("A", _c_vp), ("lda", _c_sz), ("B", _c_vp), ("ldb", _c_sz),
+                ("C", _c_vp), ("ldc", _c_sz), ("latency_ms", _c_dbl), ("id", _c_sz)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "gat_gemv_grouped": (_c_int, [_c_vp, ctypes.POINTER(GemvDesc), _c_sz]),
+    "gat_serve_cfg_default": (None, [ctypes.POINTER(ServeCfg)]),
+    "gat_server_create": (_c_vp, [_c_vp, ctypes.POINTER(ServeCfg)]),
+    "gat_server_run": (_c_int, [_c_vp, ctypes.POINTER(ServeStats)]),
+    "gat_server_kept": (_c_sz, [_c_vp]),
+    "gat_server_kept_get": (_c_int, [_c_vp, _c_sz, ctypes.POINTER(ServeReq)]),
+    "gat_server_check_alone": (_c_int, [_c_vp, ctypes.POINTER(_c_sz)]),
+    "gat_server_destroy": (None, [_c_vp]),
+    "gat_grouped_create": (_c_vp, [_c_vp, _c_int, ctypes.POINTER(GemmDesc), _c_sz, _c_int]),
+    "gat_grouped_run": (_c_int, [_c_vp]),
+    "gat_grouped_info": (_c_int, [_c_vp, ctypes.POINTER(_c_sz), ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_int)]),
+    "gat_grouped_shape": (_c_int, [_c_vp, _c_sz]),
+    "gat_grouped_destroy": (None, [_c_vp]),
+    "gat_gemm_grouped": (_c_int, [_c_vp, _c_int, ctypes.POINTER(GemmDesc), _c_sz]),
+    "gat_grouped_tune": (_c_int, [_c_vp, _c_int, ctypes.POINTER(GemmDesc), _c_sz, _c_int]),
     "gat_create": (_c_vp, [_c_int]),
     "gat_destroy": (None, [_c_vp]),
     "gat_last_error": (ctypes.c_char_p, []),
@@ -65,6 +118,9 @@ SIGNATURES = {
     "gat_axpy": (_c_int, [_c_vp, _c_int, _c_sz, _c_dbl, _c_vp, _c_vp, _c_vp]),
     "gat_axpy_compat": (_c_int, [_c_vp, _c_int, _c_sz, _c_sz, _c_dbl, _c_vp, _c_vp, _c_vp, _c_sz, _c_sz,
                                  _c_sz, _c_sz, _c_sz, _c_sz]),
+    "gat_convert_f32_bf16": (_c_int, [_c_vp, _c_vp, _c_vp, _c_sz]),
+    "gat_quantize_fp8": (_c_int, [_c_vp, _c_vp, _c_sz, _c_sz, _c_sz, _c_int, _c_vp, _c_vp]),
+    "gat_gemm_fp8": (_c_int, [_c_vp, _c_sz, _c_sz, _c_sz, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_vp, _c_sz]),
     # parameter interface
     "gat_kernel_new": (_c_vp, [_c_vp, _c_int, _c_int, _c_int, _c_int]),
     "gat_kernel_free": (None, [_c_vp]),
@@ -101,6 +157,8 @@ SIGNATURES = {
     "gat_mark_bench_avg": (_c_int, [_c_sz, ctypes.POINTER(_c_int), ctypes.POINTER(_c_dbl), _c_sz]),
     "gat_search_params": (_c_int, [_c_vp, _c_sz, _c_sz, _c_sz, _c_sz, _c_int, _c_sz, _c_sz, _c_sz, _c_int, _c_int,
                                    ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.POINTER(_c_sz), _c_sz]),
+    "gat_search_em": (_c_int, [_c_vp, ctypes.c_char_p, _c_sz, _c_sz, _c_sz, _c_sz, _c_int, _c_sz, _c_sz, _c_sz,
+                               _c_int, _c_int, _c_sz, _c_int, ctypes.POINTER(_c_sz), _c_sz]),
     "gat_tune_gemm": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, ctypes.c_char_p, _c_int,
                                ctypes.POINTER(TileCfg), ctypes.POINTER(_c_dbl)]),
     "gat_tuned_set": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, ctypes.POINTER(TileCfg)]),

@@ -301,4 +301,39 @@ int gat_axpy_compat(gat_ctx* ctx, int dtype, size_t M, size_t N, double alpha, c
                                ctx->impl.stream);
 }
 
+int gat_convert_f32_bf16(gat_ctx* ctx, const float* src, void* dst, size_t n) {
+    CTX_CHECK(ctx);
+    if (n && (!src || !dst)) {
+        set_error("convert: null operand");
+        return GAT_ERR_INVALID;
+    }
+    ctx->impl.make_current();
+    return launch_convert_f32_bf16(src, dst, n, ctx->impl.stream);
+}
+
+int gat_quantize_fp8(gat_ctx* ctx, const float* src, size_t rows, size_t cols, size_t ld, int trans, void* out,
+                     float* scales) {
+    CTX_CHECK(ctx);
+    if (!src || !out || !scales || rows > INT32_MAX || cols > INT32_MAX) {
+        set_error("quantize fp8: invalid operand");
+        return GAT_ERR_INVALID;
+    }
+    ctx->impl.make_current();
+    // element (r, c) = trans ? src[c*ld + r] : src[r*ld + c]
+    return launch_quantize_fp8(src, trans ? 1 : (int64_t)ld, trans ? (int64_t)ld : 1, (int)rows, (int)cols, out,
+                               scales, ctx->impl.stream);
+}
+
+int gat_gemm_fp8(gat_ctx* ctx, size_t M, size_t N, size_t K, double alpha, const void* A8, const float* scaleA,
+                 const void* B8, const float* scaleB, double beta, float* C, size_t ldc) {
+    CTX_CHECK(ctx);
+    GemmArgs a = make_args(0, 1, M, N, K, alpha, A8, K, 0, B8, K, 0, beta, C, ldc, 0, 1);
+    a.scaleA = scaleA;
+    a.scaleB = scaleB;
+    int rc = check_gemm_shape(a);
+    if (rc) return rc;
+    ctx->impl.make_current();
+    return ctx->impl.gemm(GAT_FP8, a, nullptr);
+}
+
 }  // extern "C"

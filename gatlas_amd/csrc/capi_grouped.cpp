@@ -1,0 +1,136 @@
+// extern "C" layer of the grouped GEMM (persistent launch, HIP graph) and
+// its per-size-bucket tile tuning.
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/gatlas.h"
+#include "capi_internal.hpp"
+#include "grouped_plan.hpp"
+
+using namespace gat;
+
+struct gat_grouped {
+    GroupedPlan plan;
+};
+
+static std::vector<GroupedDesc> to_descs(const gat_gemm_desc* d, size_t n) {
+    std::vector<GroupedDesc> v(n);
+    for (size_t i = 0; i < n; ++i)
+        v[i] = GroupedDesc{d[i].transA ? 1 : 0, d[i].transB ? 1 : 0, (int64_t)d[i].M, (int64_t)d[i].N,
+                           (int64_t)d[i].K, d[i].alpha, d[i].A, (int64_t)d[i].lda, d[i].B, (int64_t)d[i].ldb,
+                           d[i].beta, (float*)d[i].C, (int64_t)d[i].ldc};
+    return v;
+}
+
+extern "C" {
+
+gat_grouped* gat_grouped_create(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t count, int use_graph) {
+    if (!ctx || (count && !descs)) {
+        set_error("gat_grouped_create: null context or descriptors");
+        return nullptr;
+    }
+    std::unique_ptr<gat_grouped> g(new gat_grouped());
+    const std::vector<GroupedDesc> v = to_descs(descs, count);
+    if (g->plan.create(ctx->impl, dtype, v.data(), v.size(), use_graph != 0) != ST_OK) return nullptr;
+    return g.release();
+}
+
+int gat_grouped_run(gat_grouped* g) {
+    if (!g) {
+        set_error("null grouped plan");
+        return GAT_ERR_INVALID;
+    }
+    return g->plan.run();
+}
+
+int gat_grouped_info(gat_grouped* g, size_t* tiles, double* flops, int* grid) {
+    if (!g) {
+        set_error("null grouped plan");
+        return GAT_ERR_INVALID;
+    }
+    if (tiles) *tiles = g->plan.tiles();
+    if (flops) *flops = g->plan.flops();
+    if (grid) *grid = g->plan.grid();
+    return GAT_OK;
+}
+
+int gat_grouped_shape(gat_grouped* g, size_t i) { return g ? g->plan.shape(i) : -1; }
+
+void gat_grouped_destroy(gat_grouped* g) { delete g; }
+
+int gat_gemm_grouped(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t count) {
+    gat_grouped* g = gat_grouped_create(ctx, dtype, descs, count, 0);
+    if (!g) return GAT_ERR_INVALID;
+    int rc = g->plan.run();
+    if (rc == GAT_OK && hipStreamSynchronize(ctx->impl.stream) != hipSuccess) rc = GAT_ERR_HIP;
+    delete g;
+    return rc;
+}
+
+int gat_grouped_tune(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t count, int reps) {
+    if (!ctx || (count && !descs)) {
+        set_error("gat_grouped_tune: null context or descriptors");
+        return GAT_ERR_INVALID;
+    }
+    if (dtype != GAT_F32 && dtype != GAT_BF16) {
+        set_error("gat_grouped_tune: dtype must be f32 or bf16");
+        return GAT_ERR_INVALID;
+    }
+    Context& c = ctx->impl;
+    c.make_current();
+    if (reps < 1) reps = 1;
+    std::map<BucketKey, size_t> rep;  // one representative problem per size bucket
+    for (size_t i = 0; i < count; ++i) {
+        GemmArgs a;
+        a.transA = descs[i].transA ? 1 : 0;
+        a.transB = descs[i].transB ? 1 : 0;
+        a.M = (int64_t)descs[i].M;
+        a.N = (int64_t)descs[i].N;
+        a.K = (int64_t)descs[i].K;
+        rep.emplace(bucket_of(dtype, a), i);
+    }
+    int tuned = 0;
+    for (const auto& kv : rep) {
+        const gat_gemm_desc& d = descs[kv.second];
+        GemmArgs a;
+        a.transA = d.transA ? 1 : 0;
+        a.transB = d.transB ? 1 : 0;
+        a.M = (int64_t)d.M;
+        a.N = (int64_t)d.N;
+        a.K = (int64_t)d.K;
+        a.A = d.A;
+        a.lda = (int64_t)d.lda;
+        a.B = d.B;
+        a.ldb = (int64_t)d.ldb;
+        a.C = d.C;
+        a.ldc = (int64_t)d.ldc;
+        a.alpha = 1.0;
+        a.beta = 0.0;  // tuning never reads C
+        float best = 0.f;
+        int best_shape = 0;
+        for (int s = 0; s < 2; ++s) {
+            const MfmaCfg cfg = GroupedPlan::shape_cfg(dtype, s);
+            int rc = c.gemm(dtype, a, &cfg);  // warm-up
+            if (rc) return rc;
+            GAT_HIP_TRY(hipEventRecord(c.ev_start, c.stream));
+            for (int r = 0; r < reps; ++r) {
+                rc = c.gemm(dtype, a, &cfg);
+                if (rc) return rc;
+            }
+            GAT_HIP_TRY(hipEventRecord(c.ev_stop, c.stream));
+            GAT_HIP_TRY(hipEventSynchronize(c.ev_stop));
+            float ms = 0.f;
+            GAT_HIP_TRY(hipEventElapsedTime(&ms, c.ev_start, c.ev_stop));
+            if (s == 0 || ms < best) {
+                best = ms;
+                best_shape = s;
+            }
+        }
+        c.set_tuned(kv.first, GroupedPlan::shape_cfg(dtype, best_shape));
+        ++tuned;
+    }
+    return tuned;
+}
+
+}  // extern "C"

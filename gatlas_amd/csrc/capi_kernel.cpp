@@ -276,6 +276,43 @@ int gat_search_params(gat_kernel* k, size_t max_workgroup, size_t vl, size_t max
     return (int)pargs.size();
 }
 
+int gat_search_em(gat_kernel* k, const char* journal_path, size_t max_workgroup, size_t vl,
+                  size_t max_block_height, size_t max_group_size, int generalized, size_t M, size_t N, size_t K,
+                  int transA, int transB, size_t ntrials, int paranoid, size_t* best, size_t nbest) {
+    if (!k || !journal_path) return bad("gat_search_em: null kernel or journal path");
+    // Without a context the search replays the journal only; a candidate
+    // that would need a device run fails (and is journaled as such).
+    std::unique_ptr<DeviceApp> local;
+    DeviceApp* app = k->device();
+    if (!app) {
+        local.reset(new DeviceApp((Context*)nullptr));
+        app = local.get();
+    }
+    Journal journal(journal_path);
+    Bench bench(*app, *k->k, journal, true);
+    LoopOpts o;
+    o.numberTrials = ntrials ? ntrials : 1;
+    o.paranoid = paranoid != 0;
+    SearchSpec s;
+    s.maxWorkGroupSize = max_workgroup;
+    s.vl = vl;
+    s.maxBlockHeight = max_block_height;
+    s.maxGroupSize = max_group_size;
+    s.generalized = generalized != 0;
+    s.tA = transA != 0;
+    s.tB = transB != 0;
+    s.M = M;
+    s.N = N;
+    s.K = K;
+    const std::vector<size_t> b = emSearchKernel(*k->k, bench, journal, o, s);
+    if (b.empty()) {
+        set_error("gat_search_em: no good kernel");
+        return GAT_ERR_UNSUPPORTED;
+    }
+    for (size_t i = 0; i < b.size() && i < nbest; ++i) best[i] = b[i];
+    return (int)b.size();
+}
+
 // -------------------------------------------------------------- autotuner
 int gat_tune_gemm(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, size_t N, size_t K,
                   const char* journal_path, int verbose, gat_tile_cfg* best_out, double* gflops) {

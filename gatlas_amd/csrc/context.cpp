@@ -131,10 +131,16 @@ void Context::clear_tuned() {
     tuned_.clear();
 }
 
+// The FP8 kernel has one fixed shape: 2 x 2 waves of 2 x 2 32x32 blocks, K 128.
+static const MfmaCfg kFp8Cfg{2, 2, 2, 2, 128, 32};
+
 int Context::num_cfgs(int dtype) {
     switch (dtype) {
         case DT_F32: return mfma_f32_num_cfgs();
         case DT_F64: return mfma_f64_num_cfgs();
+        case DT_BF16:
+        case DT_BF16X3: return mfma_bf16_num_cfgs();
+        case DT_FP8: return 1;
     }
     return 0;
 }
@@ -143,6 +149,9 @@ MfmaCfg Context::cfg_at(int dtype, int i) {
     switch (dtype) {
         case DT_F32: return mfma_f32_cfg(i);
         case DT_F64: return mfma_f64_cfg(i);
+        case DT_BF16:
+        case DT_BF16X3: return mfma_bf16_cfg(i);
+        case DT_FP8: return kFp8Cfg;
     }
     return MfmaCfg{0, 0, 0, 0, 0, 0};
 }
@@ -151,6 +160,11 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
     switch (dtype) {
         case DT_F32: return mfma_f32_has(c);
         case DT_F64: return mfma_f64_has(c);
+        case DT_BF16:
+        case DT_BF16X3: return mfma_bf16_has(c);
+        case DT_FP8:
+            return c.waves_m == kFp8Cfg.waves_m && c.waves_n == kFp8Cfg.waves_n && c.tm == kFp8Cfg.tm &&
+                   c.tn == kFp8Cfg.tn && c.bk == kFp8Cfg.bk && c.blk == kFp8Cfg.blk;
     }
     return false;
 }
@@ -158,6 +172,12 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
 MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
     const int64_t mn = a.M < a.N ? a.M : a.N;
     if (dtype == DT_F64) return MfmaCfg{2, 2, 2, 2, 16, 16};
+    if (dtype == DT_FP8) return kFp8Cfg;
+    if (dtype == DT_BF16 || dtype == DT_BF16X3) {
+        if (mn >= 1024) return MfmaCfg{2, 2, 2, 2, 32, 32};
+        if (mn >= 256) return MfmaCfg{2, 2, 1, 1, 64, 32};
+        return MfmaCfg{1, 1, 2, 2, 64, 32};
+    }
     // FP32 (and the FP32-output forms): large square-ish -> 128x128 tiles.
     if (mn >= 1024) return MfmaCfg{2, 2, 2, 2, 32, 32};
     if (mn >= 256) return MfmaCfg{2, 2, 1, 1, 32, 32};
@@ -174,9 +194,48 @@ int Context::gemm(int dtype, const GemmArgs& a, const MfmaCfg* cfg) {
     switch (dtype) {
         case DT_F32: return launch_gemm_mfma_f32(c, a, stream);
         case DT_F64: return launch_gemm_mfma_f64(c, a, stream);
+        case DT_BF16:
+        case DT_BF16X3: return launch_gemm_mfma_bf16(dtype, c, a, stream);
+        case DT_FP8: return gemm_fp8(a);
     }
     set_error("gemm: precision form not available");
     return ST_UNSUPPORTED;
+}
+
+int Context::gemm_fp8(const GemmArgs& a) {
+    if (a.scaleA && a.scaleB) return launch_gemm_fp8(a, stream);
+    // FP32 operands: quantise op(A) and op(B)^T into the workspace first.
+    if (a.batch != 1) {
+        set_error("fp8 gemm from FP32 operands: batch must be 1");
+        return ST_INVALID;
+    }
+    const size_t M = a.M, N = a.N, K = a.K;
+    const size_t kb = (K + 127) / 128, mb = (M + 127) / 128, nb = (N + 127) / 128;
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t offB = up(M * K), offSA = offB + up(N * K), offSB = offSA + up(mb * kb * 4);
+    void* w = nullptr;
+    int rc = workspace(offSB + up(nb * kb * 4), &w);
+    if (rc) return rc;
+    unsigned char* base = (unsigned char*)w;
+    float* sA = (float*)(base + offSA);
+    float* sB = (float*)(base + offSB);
+    // op(A)[i][k]: A[i*lda + k] or A[k*lda + i]; op(B)^T[j][k] = op(B)[k][j]
+    rc = launch_quantize_fp8((const float*)a.A, a.transA ? 1 : a.lda, a.transA ? a.lda : 1, (int)M, (int)K, base,
+                             sA, stream);
+    if (rc) return rc;
+    rc = launch_quantize_fp8((const float*)a.B, a.transB ? a.ldb : 1, a.transB ? 1 : a.ldb, (int)N, (int)K,
+                             base + offB, sB, stream);
+    if (rc) return rc;
+    GemmArgs q = a;
+    q.A = base;
+    q.B = base + offB;
+    q.transA = 0;
+    q.transB = 1;
+    q.lda = a.K;
+    q.ldb = a.K;
+    q.scaleA = sA;
+    q.scaleB = sB;
+    return launch_gemm_fp8(q, stream);
 }
 
 }  // namespace gat

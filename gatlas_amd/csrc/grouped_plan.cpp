@@ -1,0 +1,139 @@
+#include "grouped_plan.hpp"
+
+#include <algorithm>
+
+namespace gat {
+
+GroupedPlan::~GroupedPlan() {
+    if (!ctx_) return;
+    ctx_->make_current();
+    hipStreamSynchronize(ctx_->stream);
+    if (exec_) hipGraphExecDestroy(exec_);
+    if (graph_) hipGraphDestroy(graph_);
+    if (d_probs_) hipFree(d_probs_);
+    if (d_work_) hipFree(d_work_);
+    if (d_counter_) hipFree(d_counter_);
+}
+
+MfmaCfg GroupedPlan::shape_cfg(int dtype, int shape) {
+    if (dtype == DT_BF16) return shape == 0 ? MfmaCfg{2, 2, 2, 2, 32, 32} : MfmaCfg{2, 2, 1, 1, 64, 32};
+    return shape == 0 ? MfmaCfg{2, 2, 2, 2, 32, 32} : MfmaCfg{2, 2, 1, 1, 32, 32};
+}
+
+int GroupedPlan::choose_shape(const Context& ctx, int dtype, const GroupedDesc& d) {
+    GemmArgs a;
+    a.transA = d.transA;
+    a.transB = d.transB;
+    a.M = d.M;
+    a.N = d.N;
+    a.K = d.K;
+    MfmaCfg c;
+    if (ctx.get_tuned(bucket_of(dtype, a), &c)) return (c.waves_m * c.tm * c.blk >= 128) ? 0 : 1;
+    // untuned: 128 x 128 tiles once both sides fill them, else 64 x 64
+    return (d.M >= 256 && d.N >= 256) ? 0 : 1;
+}
+
+int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* d, size_t n,
+                              std::vector<GroupedProblemDev>& probs, std::vector<int4>& work,
+                              std::vector<int>& shapes, double& flops, const int* force_shapes) {
+    if (dtype != DT_F32 && dtype != DT_BF16) {
+        set_error("grouped gemm: dtype must be f32 or bf16");
+        return ST_INVALID;
+    }
+    const bool bf16 = dtype == DT_BF16;
+    probs.resize(n);
+    shapes.assign(n, 0);
+    flops = 0;
+    struct Tile {
+        double cost;
+        int p, m0, n0;
+    };
+    std::vector<Tile> tl;
+    for (size_t i = 0; i < n; ++i) {
+        const GroupedDesc& g = d[i];
+        if (g.M < 0 || g.N < 0 || g.K < 0 || g.M > INT32_MAX || g.N > INT32_MAX || g.K > INT32_MAX ||
+            (g.M && g.N && !g.C) || (g.M && g.K && !g.A) || (g.N && g.K && !g.B) ||
+            g.lda < (g.transA ? g.M : g.K) || g.ldb < (g.transB ? g.K : g.N) || g.ldc < g.N) {
+            set_error("grouped gemm: invalid problem descriptor");
+            return ST_INVALID;
+        }
+        const int s = force_shapes ? force_shapes[i] : choose_shape(ctx, dtype, g);
+        shapes[i] = s;
+        const int W = bf16 ? 8 : 4;
+        const bool vec = ((uintptr_t)g.A % 16 == 0) && ((uintptr_t)g.B % 16 == 0) && g.lda % W == 0 &&
+                         g.ldb % W == 0;
+        GroupedProblemDev& p = probs[i];
+        p.A = g.A;
+        p.B = g.B;
+        p.C = g.C;
+        p.M = (int)g.M;
+        p.N = (int)g.N;
+        p.K = (int)g.K;
+        p.flags = (g.transA ? GP_TA : 0) | (g.transB ? GP_TB : 0) | (vec ? GP_VEC : 0) | (s << GP_SHAPE_SHIFT);
+        p.lda = g.lda;
+        p.ldb = g.ldb;
+        p.ldc = g.ldc;
+        p.alpha = (float)g.alpha;
+        p.beta = (float)g.beta;
+        const int T = s == 0 ? 128 : 64;
+        for (int64_t m0 = 0; m0 < g.M; m0 += T)
+            for (int64_t n0 = 0; n0 < g.N; n0 += T)
+                tl.push_back(Tile{(double)T * T * (double)(g.K + 32), (int)i, (int)m0, (int)n0});
+        flops += 2.0 * (double)g.M * (double)g.N * (double)g.K;
+    }
+    // longest tiles first; ties keep problem order
+    std::stable_sort(tl.begin(), tl.end(), [](const Tile& a, const Tile& b) { return a.cost > b.cost; });
+    work.resize(tl.size());
+    for (size_t i = 0; i < tl.size(); ++i) work[i] = make_int4(tl[i].p, tl[i].m0, tl[i].n0, 0);
+    return ST_OK;
+}
+
+int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n, bool use_graph) {
+    std::vector<GroupedProblemDev> probs;
+    std::vector<int4> work;
+    int rc = build_tables(ctx, dtype, d, n, probs, work, shapes_, flops_);
+    if (rc) return rc;
+    ctx_ = &ctx;
+    bf16_ = dtype == DT_BF16;
+    ctx.make_current();
+    nwork_ = work.size();
+    const size_t pbytes = std::max<size_t>(1, probs.size()) * sizeof(GroupedProblemDev);
+    const size_t wbytes = std::max<size_t>(1, work.size()) * sizeof(int4);
+    GAT_HIP_TRY(hipMalloc(&d_probs_, pbytes));
+    GAT_HIP_TRY(hipMalloc(&d_work_, wbytes));
+    GAT_HIP_TRY(hipMalloc((void**)&d_counter_, sizeof(int)));
+    if (!probs.empty())
+        GAT_HIP_TRY(hipMemcpyAsync(d_probs_, probs.data(), probs.size() * sizeof(GroupedProblemDev),
+                                   hipMemcpyHostToDevice, ctx.stream));
+    if (!work.empty())
+        GAT_HIP_TRY(hipMemcpyAsync(d_work_, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                   ctx.stream));
+    GAT_HIP_TRY(hipStreamSynchronize(ctx.stream));
+    // two resident work-groups per CU (LDS-limited), never more than the tiles
+    grid_ = std::max(1, std::min<int>(2 * (ctx.num_cu > 0 ? ctx.num_cu : 256), (int)std::max<size_t>(1, nwork_)));
+    if (use_graph) {
+        GAT_HIP_TRY(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+        int rc1 = launch_grouped_reset(d_counter_, ctx.stream);
+        int rc2 = launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_,
+                                      (int)nwork_, d_counter_, grid_, ctx.stream);
+        hipError_t e = hipStreamEndCapture(ctx.stream, &graph_);
+        if (rc1 || rc2) return rc1 ? rc1 : rc2;
+        GAT_HIP_TRY(e);
+        GAT_HIP_TRY(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    }
+    return ST_OK;
+}
+
+int GroupedPlan::run() {
+    ctx_->make_current();
+    if (exec_) {
+        GAT_HIP_TRY(hipGraphLaunch(exec_, ctx_->stream));
+        return ST_OK;
+    }
+    int rc = launch_grouped_reset(d_counter_, ctx_->stream);
+    if (rc) return rc;
+    return launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, (int)nwork_,
+                               d_counter_, grid_, ctx_->stream);
+}
+
+}  // namespace gat

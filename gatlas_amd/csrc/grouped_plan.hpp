@@ -1,0 +1,62 @@
+// Host side of the persistent grouped GEMM: problem table, cost-ordered
+// tile list, per-bucket tile choice, and the optional HIP graph holding
+// the (counter reset, persistent kernel) pair.
+#pragma once
+
+#include <vector>
+
+#include "context.hpp"
+#include "kernels/grouped.hpp"
+
+namespace gat {
+
+struct GroupedDesc {
+    int transA, transB;
+    int64_t M, N, K;
+    double alpha;
+    const void* A;
+    int64_t lda;
+    const void* B;
+    int64_t ldb;
+    double beta;
+    float* C;
+    int64_t ldc;
+};
+
+class GroupedPlan {
+public:
+    ~GroupedPlan();
+    // dtype DT_F32 (FP32 operands) or DT_BF16 (bf16 operands), FP32 output.
+    int create(Context& ctx, int dtype, const GroupedDesc* d, size_t n, bool use_graph);
+    int run();  // one persistent launch (graph replay when captured)
+    size_t tiles() const { return nwork_; }
+    double flops() const { return flops_; }
+    int grid() const { return grid_; }
+    // tile shape chosen for problem i (0 = 128x128, 1 = 64x64)
+    int shape(size_t i) const { return shapes_[i]; }
+
+    // shape for one problem: the context's tuned choice for its size bucket
+    // when one exists, else a size rule
+    static int choose_shape(const Context& ctx, int dtype, const GroupedDesc& d);
+    static MfmaCfg shape_cfg(int dtype, int shape);
+    // Device descriptors and the cost-ordered (problem, m0, n0) tile list
+    // for n problems; also the tile shape per problem and the total flops.
+    static int build_tables(const Context& ctx, int dtype, const GroupedDesc* d, size_t n,
+                            std::vector<GroupedProblemDev>& probs, std::vector<int4>& work,
+                            std::vector<int>& shapes, double& flops, const int* force_shapes = nullptr);
+
+private:
+    Context* ctx_ = nullptr;
+    int bf16_ = 0;
+    void* d_probs_ = nullptr;
+    void* d_work_ = nullptr;
+    int* d_counter_ = nullptr;
+    size_t nwork_ = 0;
+    int grid_ = 0;
+    double flops_ = 0;
+    std::vector<int> shapes_;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+};
+
+}  // namespace gat

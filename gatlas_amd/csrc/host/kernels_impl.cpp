@@ -163,6 +163,7 @@ public:
 // ======================================================================
 class KMatmulMfma : public KernelBaseMatmul, public KernelTag {
     int hA_ = -1, hB_ = -1, hC_ = -1;
+    int hA2_ = -1, hB2_ = -1, hSA_ = -1, hSB_ = -1;  // converted operands (BF16 / FP8)
     bool paranoid_ = false;
     bool f64_;
     double alpha_ = 1, beta_ = 1;
@@ -261,6 +262,7 @@ public:
         const size_t es = f64_ ? 8 : 4;
         if (hA_ < 0 || hB_ < 0 || hC_ < 0 || dimChanged() || packedChanged() || app.bytes(hC_) != P * M * N * es) {
             app.releaseBuffers();
+            hA2_ = hB2_ = hSA_ = hSB_ = -1;
             if (f64_) {
                 hA_ = app.createBuffer<double>(P * M * K, "matA", 1.0);
                 hB_ = app.createBuffer<double>(P * K * N, "matB", 1.0);
@@ -286,9 +288,41 @@ public:
             else prepare<float>(app);
             if (!app.syncToDevice(hA_) || !app.syncToDevice(hB_) || !app.syncToDevice(hC_)) return false;
         }
-        return true;
+        return convertInputs(app);
     }
-    bool build(DeviceApp&) override { return validParams(); }
+    // BF16: device bf16 copies of A and B; FP8: block-quantised op(A) (M x K)
+    // and op(B)^T (N x K) with their scales.  Done here, outside the timed
+    // launch, as the reference uploads its operands in setArgs.
+    bool convertInputs(DeviceApp& app) {
+        if (dtype != DT_BF16 && dtype != DT_FP8) return true;
+        const size_t P = packedCalc(), M = dimM(), N = dimN(), K = dimK();
+        if (dtype == DT_BF16) {
+            if (hA2_ < 0) hA2_ = app.createRaw(P * M * K * 2, "matA.bf16");
+            if (hB2_ < 0) hB2_ = app.createRaw(P * K * N * 2, "matB.bf16");
+            if (hA2_ < 0 || hB2_ < 0) return false;
+            return launch_convert_f32_bf16((const float*)app.devicePtr(hA_), app.devicePtr(hA2_), P * M * K,
+                                           app.stream()) == ST_OK &&
+                   launch_convert_f32_bf16((const float*)app.devicePtr(hB_), app.devicePtr(hB2_), P * K * N,
+                                           app.stream()) == ST_OK &&
+                   app.wait();
+        }
+        if (P != 1) return false;
+        const size_t kb = (K + 127) / 128;
+        if (hA2_ < 0) hA2_ = app.createRaw(M * K, "matA.fp8");
+        if (hB2_ < 0) hB2_ = app.createRaw(N * K, "matB.fp8");
+        if (hSA_ < 0) hSA_ = app.createRaw(((M + 127) / 128) * kb * 4, "scaleA");
+        if (hSB_ < 0) hSB_ = app.createRaw(((N + 127) / 128) * kb * 4, "scaleB");
+        if (hA2_ < 0 || hB2_ < 0 || hSA_ < 0 || hSB_ < 0) return false;
+        const int64_t lda = transposeA() ? M : K, ldb = transposeB() ? K : N;
+        return launch_quantize_fp8((const float*)app.devicePtr(hA_), transposeA() ? 1 : lda, transposeA() ? lda : 1,
+                                   (int)M, (int)K, app.devicePtr(hA2_), (float*)app.devicePtr(hSA_),
+                                   app.stream()) == ST_OK &&
+               launch_quantize_fp8((const float*)app.devicePtr(hB_), transposeB() ? ldb : 1, transposeB() ? 1 : ldb,
+                                   (int)N, (int)K, app.devicePtr(hB2_), (float*)app.devicePtr(hSB_),
+                                   app.stream()) == ST_OK &&
+               app.wait();
+    }
+    bool build(DeviceApp&) override { return validParams() && (dtype != DT_FP8 || packedCalc() == 1); }
     bool enqueue(DeviceApp& app) override {
         GemmArgs a;
         a.transA = transposeA();
@@ -308,6 +342,19 @@ public:
         a.batch = (int)packedCalc();
         a.alpha = generalizedMatmul() ? alpha_ : 1.0;
         a.beta = generalizedMatmul() ? beta_ : 0.0;
+        if (dtype == DT_BF16) {
+            a.A = app.devicePtr(hA2_);
+            a.B = app.devicePtr(hB2_);
+        } else if (dtype == DT_FP8) {
+            a.A = app.devicePtr(hA2_);
+            a.B = app.devicePtr(hB2_);
+            a.transA = 0;
+            a.transB = 1;
+            a.lda = a.K;
+            a.ldb = a.K;
+            a.scaleA = (const float*)app.devicePtr(hSA_);
+            a.scaleB = (const float*)app.devicePtr(hSB_);
+        }
         const MfmaCfg c = cfg();
         return app.ctx().gemm(dtype, a, &c) == ST_OK;
     }

@@ -352,4 +352,106 @@ std::vector<size_t> emSearch(KernelInterface& k, Bench& bench, Journal& journal,
     return best;
 }
 
+std::vector<std::vector<size_t>> allCandidates(KernelInterface& k, const SearchSpec& s) {
+    const int var = kernel_variant(&k);
+    if (auto* m = as_matmul(&k)) {
+        if (var == VAR_MFMA)
+            return getParamsMatmulMfma(*m, s.generalized, s.M, s.N, s.K, s.tA, s.tB, std::vector<size_t>(), FREE);
+        return getParamsMatmul(*m, s.maxWorkGroupSize, s.vl, s.maxBlockHeight, s.maxGroupSize, s.generalized, s.M,
+                               s.N, s.K, s.tA, s.tB, FREE, FREE, FREE);
+    }
+    if (auto* v = as_matvec(&k)) {
+        if (var == VAR_NATIVE) return getParamsMatvecNative(*v, s.generalized, s.M, s.N, s.tA, FREE, FREE, FREE);
+        return getParamsMatvec(*v, s.maxWorkGroupSize, s.vl, s.maxBlockHeight, s.maxGroupSize, s.generalized, s.M,
+                               s.N, s.tA, FREE, FREE, FREE);
+    }
+    if (auto* x = as_saxpy(&k)) {
+        if (var == VAR_NATIVE) return getParamsSaxpyNative(*x, s.M, s.N, FREE, FREE, FREE);
+        return getParamsSaxpy(*x, s.maxWorkGroupSize, s.vl, s.maxBlockHeight, s.maxGroupSize, s.M, s.N, FREE, FREE,
+                              FREE, FREE, FREE);
+    }
+    return {};
+}
+
+std::vector<size_t> emSearchKernel(KernelInterface& k, Bench& bench, Journal& journal, const LoopOpts& o,
+                                   const SearchSpec& s) {
+    const int var = kernel_variant(&k);
+    std::vector<size_t> best;
+    if (auto* m = as_matmul(&k)) {
+        // benchMatmul.cpp:536-627: step 0 fixes the extra parameter, step 1
+        // fixes the work group and blocking.
+        const size_t total = m->totalVariations();
+        size_t seedXp = 0;
+        auto enumerate = [&](int step, const std::vector<size_t>& b) {
+            if (var == VAR_MFMA) {
+                if (step == 0)
+                    return getParamsMatmulMfma(*m, s.generalized, s.M, s.N, s.K, s.tA, s.tB, std::vector<size_t>(),
+                                               (long)(b.empty() ? seedXp : b[11]));
+                return getParamsMatmulMfma(*m, s.generalized, s.M, s.N, s.K, s.tA, s.tB,
+                                           std::vector<size_t>{b[7], b[8], b[9], b[10]}, FREE);
+            }
+            if (step == 0)
+                return getParamsMatmul(*m, s.maxWorkGroupSize, s.vl, s.maxBlockHeight, s.maxGroupSize, s.generalized,
+                                       s.M, s.N, s.K, s.tA, s.tB, FREE, FREE, (long)(b.empty() ? seedXp : b[11]));
+            return getParamsMatmul(*m, s.maxWorkGroupSize, s.vl, s.maxBlockHeight, s.maxGroupSize, s.generalized,
+                                   s.M, s.N, s.K, s.tA, s.tB, (long)b[7], (long)b[9], FREE);
+        };
+        auto bump = [&](std::vector<size_t>& b) {
+            if (!b.empty()) {
+                b[11] = (b[11] + 1) % total;
+                return true;
+            }
+            return ++seedXp < total;
+        };
+        best = emSearch(k, bench, journal, o, {7, 8, 9, 10, 11}, enumerate, bump);
+    } else if (auto* v = as_matvec(&k)) {
+        // benchMatvec.cpp:521-612
+        const size_t total = v->totalVariations();
+        size_t seedXp = 0;
+        auto params = [&](long g, long b, long x) {
+            if (var == VAR_NATIVE) return getParamsMatvecNative(*v, s.generalized, s.M, s.N, s.tA, g, b, x);
+            return getParamsMatvec(*v, s.maxWorkGroupSize, s.vl, s.maxBlockHeight, s.maxGroupSize, s.generalized, s.M,
+                                   s.N, s.tA, g, b, x);
+        };
+        auto enumerate = [&](int step, const std::vector<size_t>& b) {
+            if (step == 0) return params(FREE, FREE, (long)(b.empty() ? seedXp : b[8]));
+            return params((long)b[5], (long)b[6], FREE);
+        };
+        auto bump = [&](std::vector<size_t>& b) {
+            if (!b.empty()) {
+                b[8] = (b[8] + 1) % total;
+                return true;
+            }
+            return ++seedXp < total;
+        };
+        best = emSearch(k, bench, journal, o, {5, 6, 8}, enumerate, bump);
+    } else if (auto* x = as_saxpy(&k)) {
+        // benchSaxpy.cpp:498-584: step 0 frees the work group with blocking
+        // and extra parameter fixed; step 1 frees blocking and extra parameter.
+        auto params = [&](long gH, long gW, long bh, long bw, long xp) {
+            if (var == VAR_NATIVE) return getParamsSaxpyNative(*x, s.M, s.N, gH, bh, xp);
+            return getParamsSaxpy(*x, s.maxWorkGroupSize, s.vl, s.maxBlockHeight, s.maxGroupSize, s.M, s.N, gH, gW,
+                                  bh, bw, xp);
+        };
+        const long initBH = var == VAR_NATIVE ? 4 : 1;
+        const long initBW = (s.N == s.vl) ? 0 : (long)s.vl;
+        auto enumerate = [&](int step, const std::vector<size_t>& b) {
+            if (step == 0) {
+                if (b.empty()) return params(FREE, FREE, initBH, initBW, 0);
+                return params(FREE, FREE, (long)b[5], (long)b[6], (long)b[8]);
+            }
+            return params((long)b[3], (long)b[4], FREE, FREE, FREE);
+        };
+        best = emSearch(k, bench, journal, o, {3, 4, 5, 6, 8}, enumerate, nullptr);
+    }
+    if (!best.empty() && o.numberTrials > 1) {
+        std::vector<std::vector<size_t>> pargs{best};
+        std::vector<bool> ok(1, true);
+        std::vector<double> avg(1, 0.0);
+        journal.loadMemo();
+        mainLoop(k, bench, journal, pargs, ok, avg, o);
+    }
+    return best;
+}
+
 }  // namespace gat

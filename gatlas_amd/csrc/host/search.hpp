@@ -63,4 +63,20 @@ std::vector<size_t> emSearch(KernelInterface& k, Bench& bench, Journal& journal,
                                  enumerate,
                              const std::function<bool(std::vector<size_t>&)>& bumpExtra);
 
+// Search problem shared by the command-line tools and the C ABI.
+struct SearchSpec {
+    size_t maxWorkGroupSize = 1024;
+    size_t vl = 1, maxBlockHeight = 10, maxGroupSize = 10;
+    bool generalized = false, tA = false, tB = false;
+    size_t M = 0, N = 0, K = 0;
+};
+
+// Full EM search of one kernel (any family/variant) for `spec`, followed by
+// the final multi-trial average when o.numberTrials > 1.  Candidates whose
+// journal record is complete are replayed without touching the device.
+std::vector<size_t> emSearchKernel(KernelInterface& k, Bench& bench, Journal& journal, const LoopOpts& o,
+                                   const SearchSpec& spec);
+// Brute-force enumeration of all candidates (no -e), same conventions.
+std::vector<std::vector<size_t>> allCandidates(KernelInterface& k, const SearchSpec& spec);
+
 }  // namespace gat

@@ -24,8 +24,9 @@ struct GemmArgs {
     int64_t ldc = 0, strideC = 0;
     double alpha = 1.0, beta = 0.0;
     int batch = 1;
-    // FP8 only: per-128x128-block FP32 scales of op(A) (ceil(M/128) x ceil(K/128),
-    // row-major) and op(B) (ceil(K/128) x ceil(N/128), row-major).
+    // FP8 only: per-128x128-block FP32 scales of op(A) stored M x K
+    // (ceil(M/128) x ceil(K/128), row-major) and of op(B)^T stored N x K
+    // (ceil(N/128) x ceil(K/128), row-major).
     const float* scaleA = nullptr;
     const float* scaleB = nullptr;
 };
@@ -97,5 +98,20 @@ MfmaCfg mfma_f32_cfg(int i);
 int mfma_f64_num_cfgs();
 MfmaCfg mfma_f64_cfg(int i);
 bool compat_has(int dtype, const CompatCfg& cfg);
+
+// BF16 forms (gemm_mfma_bf16.hip): dtype DT_BF16 (bf16 inputs) or
+// DT_BF16X3 (FP32 inputs split into bf16 hi/lo, three products).
+int launch_gemm_mfma_bf16(int dtype, const MfmaCfg& cfg, const GemmArgs& a, hipStream_t st);
+int mfma_bf16_num_cfgs();
+MfmaCfg mfma_bf16_cfg(int i);
+bool mfma_bf16_has(const MfmaCfg& cfg);
+int launch_convert_f32_bf16(const float* x, void* y, size_t n, hipStream_t st);
+
+// FP8 e4m3 (gemm_fp8.hip): block quantisation of a rows x cols operand whose
+// element (r, c) is src[r*rs + c*cs] into row-major bytes + block scales,
+// and the scaled GEMM (A8 = op(A) as M x K, B8 = op(B)^T as N x K).
+int launch_quantize_fp8(const float* src, int64_t rs, int64_t cs, int rows, int cols, void* out, float* scales,
+                        hipStream_t st);
+int launch_gemm_fp8(const GemmArgs& a, hipStream_t st);
 
 }  // namespace gat

@@ -1,0 +1,263 @@
+// One macro tile of the BF16 matrix-core GEMMs (v_mfma_f32_32x32x16_bf16),
+// shared by the per-problem kernel and the persistent grouped kernel.
+//
+//   MODE_BF16: A and B are BF16 arrays.
+//   MODE_X3:   A and B are FP32; while a K tile is staged, every element is
+//              split into hi = bf16(a) and lo = bf16(a - hi) (both rounded to
+//              nearest even; a - hi is exact in FP32), and the product is
+//              accumulated as hi*hi + hi*lo + lo*hi ("split-BF16x3").  The
+//              dropped lo*lo term and the split residuals are below
+//              2^-16 |a||b| per product.
+//
+// Decomposition as in the FP32 MFMA kernel: WM x WN wavefronts per group,
+// TM x TN 32x32 blocks per wave, K staged BK deep through double-buffered
+// LDS; every operand plane is stored k-contiguous ([row][k], row stride
+// BK+8 bf16 = a multiple of 16 bytes) so one ds_read_b128 hands a lane the
+// eight k values of its MFMA operand: lane l holds row l%32 and k
+// kk + 8*(l/32) .. +7 of a 16-deep step.
+#pragma once
+
+#include "mfma_common.hpp"
+
+namespace gat {
+
+enum { MODE_BF16 = 0, MODE_X3 = 1 };
+
+__device__ __forceinline__ void split_bf16(float a, __bf16& hi, __bf16& lo) {
+    hi = (__bf16)a;
+    lo = (__bf16)(a - (float)hi);
+}
+
+// Global -> register -> LDS staging of a ROWS x BK tile for one operand.
+//   direct (k-contiguous source): element (r, k) at src[(r0 + r) * ld + k0 + k]
+//   transposed (row-contiguous):  element (r, k) at src[(k0 + k) * ld + r0 + r]
+template <int MODE, int ROWS, int BK, int NT, bool TRANS>
+struct StageIO;
+
+// ---- BF16 inputs: 8 bf16 (16 bytes) per vector load
+template <int ROWS, int BK, int NT, bool TRANS>
+struct StageIO<MODE_BF16, ROWS, BK, NT, TRANS> {
+    typedef __bf16 src_t;
+    static constexpr int W = 8;
+    static constexpr int NV = ROWS * BK / W / NT;
+    static_assert(NV >= 1 && NV * W * NT == ROWS * BK, "tile must split into 16-byte vectors per thread");
+    static constexpr int LDK = BK + 8;
+    u32x4 reg[NV];
+
+    __device__ __forceinline__ void load(const __bf16* __restrict__ src, int64_t ld, int r0, int k0, int rlim,
+                                         int klim, bool full, int tid) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int idx = q * NT + tid;
+            int gr, gk;
+            if (!TRANS) {
+                gr = r0 + idx / (BK / W);
+                gk = k0 + (idx % (BK / W)) * W;
+            } else {
+                gk = k0 + idx % BK;
+                gr = r0 + (idx / BK) * W;
+            }
+            const __bf16* p = !TRANS ? src + (int64_t)gr * ld + gk : src + (int64_t)gk * ld + gr;
+            if (full) {
+                reg[q] = *(const u32x4*)p;
+            } else {
+                __bf16 t[W];
+#pragma unroll
+                for (int e = 0; e < W; ++e) {
+                    const bool in = !TRANS ? (gr < rlim && gk + e < klim) : (gk < klim && gr + e < rlim);
+                    t[e] = in ? p[e] : (__bf16)0.0f;
+                }
+                reg[q] = *(const u32x4*)t;
+            }
+        }
+    }
+    __device__ __forceinline__ void store(__bf16* hi, __bf16* /*lo*/, int tid) const {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int idx = q * NT + tid;
+            if (!TRANS) {
+                const int r = idx / (BK / W), k = (idx % (BK / W)) * W;
+                *(u32x4*)(hi + r * LDK + k) = reg[q];
+            } else {
+                const int k = idx % BK, r = (idx / BK) * W;
+                const __bf16* v = (const __bf16*)&reg[q];
+#pragma unroll
+                for (int e = 0; e < W; ++e) hi[(r + e) * LDK + k] = v[e];
+            }
+        }
+    }
+};
+
+// ---- FP32 inputs split into hi/lo planes: 4 floats per vector load
+template <int ROWS, int BK, int NT, bool TRANS>
+struct StageIO<MODE_X3, ROWS, BK, NT, TRANS> {
+    typedef float src_t;
+    static constexpr int W = 4;
+    static constexpr int NV = ROWS * BK / W / NT;
+    static_assert(NV >= 1 && NV * W * NT == ROWS * BK, "tile must split into 16-byte vectors per thread");
+    static constexpr int LDK = BK + 8;
+    f4v reg[NV];
+
+    __device__ __forceinline__ void load(const float* __restrict__ src, int64_t ld, int r0, int k0, int rlim,
+                                         int klim, bool full, int tid) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int idx = q * NT + tid;
+            int gr, gk;
+            if (!TRANS) {
+                gr = r0 + idx / (BK / W);
+                gk = k0 + (idx % (BK / W)) * W;
+            } else {
+                gk = k0 + idx % BK;
+                gr = r0 + (idx / BK) * W;
+            }
+            const float* p = !TRANS ? src + (int64_t)gr * ld + gk : src + (int64_t)gk * ld + gr;
+            if (full) {
+                reg[q] = *(const f4v*)p;
+            } else {
+#pragma unroll
+                for (int e = 0; e < W; ++e) {
+                    const bool in = !TRANS ? (gr < rlim && gk + e < klim) : (gk < klim && gr + e < rlim);
+                    reg[q][e] = in ? p[e] : 0.0f;
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ void store(__bf16* hi, __bf16* lo, int tid) const {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int idx = q * NT + tid;
+            __bf16 h[W], l[W];
+#pragma unroll
+            for (int e = 0; e < W; ++e) split_bf16(reg[q][e], h[e], l[e]);
+            if (!TRANS) {
+                const int r = idx / (BK / W), k = (idx % (BK / W)) * W;
+                *(u32x2*)(hi + r * LDK + k) = *(const u32x2*)h;
+                *(u32x2*)(lo + r * LDK + k) = *(const u32x2*)l;
+            } else {
+                const int k = idx % BK, r = (idx / BK) * W;
+#pragma unroll
+                for (int e = 0; e < W; ++e) {
+                    hi[(r + e) * LDK + k] = h[e];
+                    lo[(r + e) * LDK + k] = l[e];
+                }
+            }
+        }
+    }
+};
+
+template <int WM, int WN, int TM, int TN, int BK, int MODE>
+struct Bf16Shape {
+    static constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    static constexpr size_t SMEM = 2 * (size_t)(MODE == MODE_X3 ? 2 : 1) * (BM + BN) * (BK + 8) * sizeof(__bf16);
+};
+
+// Computes the BM x BN tile at (m0, n0); called by all WM*WN*64 threads of
+// the group, ends after a barrier (LDS free for the next tile).
+template <int WM, int WN, int TM, int TN, int BK, int MODE, bool TA, bool TB>
+__device__ __forceinline__ void bf16_tile(const void* __restrict__ Av, const void* __restrict__ Bv,
+                                          float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb,
+                                          int64_t ldc, float alpha, float beta, int m0, int n0, bool vec_ok,
+                                          __bf16* smem) {
+    constexpr int NT = WM * WN * 64;
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    constexpr int LDK = BK + 8;
+    constexpr int NPL = MODE == MODE_X3 ? 2 : 1;  // operand planes (hi[, lo])
+    static_assert(BK % 16 == 0, "BK must cover whole 16-deep MFMA steps");
+    typedef StageIO<MODE, BM, BK, NT, TA> IoA;
+    typedef StageIO<MODE, BN, BK, NT, !TB> IoB;
+    typedef typename IoA::src_t src_t;
+    constexpr int PLANE_A = BM * LDK, PLANE_B = BN * LDK;
+    constexpr int BUF = NPL * (PLANE_A + PLANE_B);
+    const src_t* A = (const src_t*)Av;
+    const src_t* B = (const src_t*)Bv;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const bool full_mn = vec_ok && (m0 + BM <= M) && (n0 + BN <= N);
+
+    f16v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    IoA ioa;
+    IoB iob;
+    const int KT = (K + BK - 1) / BK;
+    auto gload = [&](int t) {
+        const int k0 = t * BK;
+        const bool full = full_mn && (k0 + BK <= K);
+        ioa.load(A, lda, m0, k0, M, K, full, tid);
+        iob.load(B, ldb, n0, k0, N, K, full, tid);
+    };
+    auto sstore = [&](__bf16* buf) {
+        ioa.store(buf, buf + PLANE_A + PLANE_B, tid);
+        iob.store(buf + PLANE_A, buf + 2 * PLANE_A + PLANE_B, tid);
+    };
+
+    gload(0);
+    sstore(smem);
+    __syncthreads();
+
+    const int arow = wm * TM * 32 + (lane & 31);
+    const int brow = wn * TN * 32 + (lane & 31);
+    const int kq = 8 * (lane >> 5);
+
+    for (int t = 0; t < KT; ++t) {
+        const __bf16* Ah = smem + (t & 1) * BUF;
+        const __bf16* Bh = Ah + PLANE_A;
+        const __bf16* Al = Ah + PLANE_A + PLANE_B;
+        const __bf16* Bl = Al + PLANE_A;
+        if (t + 1 < KT) gload(t + 1);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            bf16x8 ah[TM], bh[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) ah[i] = *(const bf16x8*)(Ah + (arow + i * 32) * LDK + kk + kq);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bh[j] = *(const bf16x8*)(Bh + (brow + j * 32) * LDK + kk + kq);
+            if (MODE == MODE_X3) {
+                bf16x8 al[TM], bl[TN];
+#pragma unroll
+                for (int i = 0; i < TM; ++i) al[i] = *(const bf16x8*)(Al + (arow + i * 32) * LDK + kk + kq);
+#pragma unroll
+                for (int j = 0; j < TN; ++j) bl[j] = *(const bf16x8*)(Bl + (brow + j * 32) * LDK + kk + kq);
+                // small terms first, then the leading product
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+        if (t + 1 < KT) sstore(smem + ((t + 1) & 1) * BUF);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * TM * 32 + i * 32 + mfma32_row(lane, r);
+                const int col = n0 + wn * TN * 32 + j * 32 + mfma32_col(lane);
+                if (row < M && col < N) {
+                    float* p = C + (int64_t)row * ldc + col;
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.0f) v += beta * *p;
+                    *p = v;
+                }
+            }
+}
+
+}  // namespace gat

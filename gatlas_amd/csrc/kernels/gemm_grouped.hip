@@ -1,0 +1,109 @@
+// Persistent grouped GEMM: many independent problems of different shapes
+// and transposes in one launch.
+//
+// The host builds a work list of (problem, m0, n0) tiles sorted by
+// decreasing tile cost (tile area x K), so the longest tiles start first.
+// A grid of about two work-groups per CU loops: one work-item takes the
+// next list index from a device counter (atomicAdd), the group computes that
+// tile with the tile routine of the problem's precision, shape and transpose
+// pair, and every group leaves the loop once the list is exhausted.  The
+// counter is reset by a one-thread kernel issued before each launch (kernel
+// nodes only, so the pair can be captured in a HIP graph and replayed).
+//
+// Tile shapes: 0 = 128 x 128 (2 x 2 waves of 2 x 2 32x32 blocks),
+//              1 = 64 x 64 (2 x 2 waves of one 32x32 block).
+#include "bf16_tile.hpp"
+#include "grouped.hpp"
+#include "sgemm_tile.hpp"
+
+namespace gat {
+
+__global__ void grouped_reset_kernel(int* counter) { *counter = 0; }
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void f32_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0, float* smem) {
+    const bool vec = (p.flags & GP_VEC) != 0;
+    if (shape == 0)
+        sgemm_tile<2, 2, 2, 2, 32, 32, TA, TB>((const float*)p.A, (const float*)p.B, p.C, p.M, p.N, p.K, p.lda,
+                                               p.ldb, p.ldc, p.alpha, p.beta, m0, n0, vec, smem);
+    else
+        sgemm_tile<2, 2, 1, 1, 32, 32, TA, TB>((const float*)p.A, (const float*)p.B, p.C, p.M, p.N, p.K, p.lda,
+                                               p.ldb, p.ldc, p.alpha, p.beta, m0, n0, vec, smem);
+}
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0,
+                                                __bf16* smem) {
+    const bool vec = (p.flags & GP_VEC) != 0;
+    if (shape == 0)
+        bf16_tile<2, 2, 2, 2, 32, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+                                                     p.beta, m0, n0, vec, smem);
+    else
+        bf16_tile<2, 2, 1, 1, 64, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+                                                     p.beta, m0, n0, vec, smem);
+}
+
+template <int BF16>
+__global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupedProblemDev* __restrict__ probs,
+                                                           const int4* __restrict__ work, int nwork,
+                                                           int* __restrict__ counter) {
+    extern __shared__ float gsmem[];
+    __shared__ int next;
+    for (;;) {
+        __syncthreads();  // every work-item has read the previous `next`
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int w = next;
+        if (w >= nwork) break;  // uniform across the group
+        const int4 t = work[w];
+        const GroupedProblemDev p = probs[t.x];
+        const int shape = (p.flags >> GP_SHAPE_SHIFT) & 15;
+        const int tr = p.flags & (GP_TA | GP_TB);
+        if (!BF16) {
+            if (tr == 0) f32_tile_shape<false, false>(p, shape, t.y, t.z, gsmem);
+            else if (tr == GP_TA) f32_tile_shape<true, false>(p, shape, t.y, t.z, gsmem);
+            else if (tr == GP_TB) f32_tile_shape<false, true>(p, shape, t.y, t.z, gsmem);
+            else f32_tile_shape<true, true>(p, shape, t.y, t.z, gsmem);
+        } else {
+            __bf16* hs = (__bf16*)gsmem;
+            if (tr == 0) bf16_tile_shape<false, false>(p, shape, t.y, t.z, hs);
+            else if (tr == GP_TA) bf16_tile_shape<true, false>(p, shape, t.y, t.z, hs);
+            else if (tr == GP_TB) bf16_tile_shape<false, true>(p, shape, t.y, t.z, hs);
+            else bf16_tile_shape<true, true>(p, shape, t.y, t.z, hs);
+        }
+    }
+}
+
+size_t grouped_smem_bytes(int bf16) {
+    if (!bf16) {
+        const size_t a = SgemmShape<2, 2, 2, 2, 32, 32>::SMEM, b = SgemmShape<2, 2, 1, 1, 32, 32>::SMEM;
+        return a > b ? a : b;
+    }
+    const size_t a = Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM, b = Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM;
+    return a > b ? a : b;
+}
+
+int launch_grouped_reset(int* counter, hipStream_t st) {
+    hipLaunchKernelGGL(grouped_reset_kernel, dim3(1), dim3(1), 0, st, counter);
+    GAT_HIP_TRY(hipGetLastError());
+    return ST_OK;
+}
+
+int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* work, int nwork, int* counter,
+                        int grid, hipStream_t st) {
+    const size_t smem = grouped_smem_bytes(bf16);
+    const void* f = bf16 ? (const void*)grouped_gemm_kernel<1> : (const void*)grouped_gemm_kernel<0>;
+    static bool attr_set[2] = {false, false};
+    if (!attr_set[bf16 ? 1 : 0] && smem > 65536) {
+        GAT_HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        attr_set[bf16 ? 1 : 0] = true;
+    }
+    if (bf16)
+        hipLaunchKernelGGL(grouped_gemm_kernel<1>, dim3(grid), dim3(256), smem, st, probs, work, nwork, counter);
+    else
+        hipLaunchKernelGGL(grouped_gemm_kernel<0>, dim3(grid), dim3(256), smem, st, probs, work, nwork, counter);
+    GAT_HIP_TRY(hipGetLastError());
+    return ST_OK;
+}
+
+}  // namespace gat

@@ -1,0 +1,107 @@
+// BF16 matrix-core GEMMs on gfx950: one work-group per macro tile in
+// XCD-aware supertile order; the tile computation is bf16_tile
+// (bf16_tile.hpp), shared with the persistent grouped kernel.
+#include "bf16_tile.hpp"
+
+namespace gat {
+
+template <int WM, int WN, int TM, int TN, int BK, int MODE, bool TA, bool TB>
+__global__ __launch_bounds__(WM* WN * 64) void gemm_bf16_kernel(
+    const void* __restrict__ Av, const void* __restrict__ Bv, float* __restrict__ C, int M, int N, int K,
+    int64_t lda, int64_t ldb, int64_t ldc, int64_t sA, int64_t sB, int64_t sC, float alpha, float beta,
+    int tiles_m, int tiles_n, int group_m, int vec_ok) {
+    typedef Bf16Shape<WM, WN, TM, TN, BK, MODE> S;
+    constexpr size_t ES = MODE == MODE_X3 ? sizeof(float) : sizeof(__bf16);
+    extern __shared__ __bf16 smemh[];
+    const char* A = (const char*)Av + (int64_t)blockIdx.y * sA * ES;
+    const char* B = (const char*)Bv + (int64_t)blockIdx.y * sB * ES;
+    C += (int64_t)blockIdx.y * sC;
+    int tmi, tni;
+    mfma_tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, group_m, tmi, tni);
+    bf16_tile<WM, WN, TM, TN, BK, MODE, TA, TB>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, tmi * S::BM,
+                                               tni * S::BN, vec_ok != 0, smemh);
+}
+
+// Instantiated configurations (WM, WN, TM, TN, BK); blocks are 32x32.
+#define GAT_BF16_CFGS(X) \
+    X(2, 2, 2, 2, 32)    \
+    X(2, 2, 2, 2, 64)    \
+    X(2, 2, 4, 2, 32)    \
+    X(2, 2, 2, 4, 32)    \
+    X(2, 2, 1, 1, 64)    \
+    X(1, 1, 2, 2, 64)
+
+static const MfmaCfg kBf16Cfgs[] = {
+#define X(wm, wn, tm, tn, bk) {wm, wn, tm, tn, bk, 32},
+    GAT_BF16_CFGS(X)
+#undef X
+};
+
+int mfma_bf16_num_cfgs() { return (int)(sizeof(kBf16Cfgs) / sizeof(kBf16Cfgs[0])); }
+MfmaCfg mfma_bf16_cfg(int i) { return kBf16Cfgs[i]; }
+bool mfma_bf16_has(const MfmaCfg& c) {
+    for (const MfmaCfg& k : kBf16Cfgs)
+        if (k.waves_m == c.waves_m && k.waves_n == c.waves_n && k.tm == c.tm && k.tn == c.tn && k.bk == c.bk &&
+            k.blk == c.blk)
+            return true;
+    return false;
+}
+
+template <int WM, int WN, int TM, int TN, int BK, int MODE>
+static int launch_bf16_one(const GemmArgs& a, hipStream_t st) {
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    constexpr int NPL = MODE == MODE_X3 ? 2 : 1;
+    constexpr size_t smem = 2 * (size_t)NPL * (BM + BN) * (BK + 8) * sizeof(__bf16);
+    const int tiles_m = (int)((a.M + BM - 1) / BM), tiles_n = (int)((a.N + BN - 1) / BN);
+    const int W = MODE == MODE_X3 ? 4 : 8;  // elements per 16-byte vector
+    const int vec_ok = ((uintptr_t)a.A % 16 == 0) && ((uintptr_t)a.B % 16 == 0) && (a.lda % W == 0) &&
+                       (a.ldb % W == 0) && (a.strideA % W == 0) && (a.strideB % W == 0);
+    typedef void (*kfn)(const void*, const void*, float*, int, int, int, int64_t, int64_t, int64_t, int64_t, int64_t,
+                        int64_t, float, float, int, int, int, int);
+    kfn f;
+    if (!a.transA && !a.transB) f = gemm_bf16_kernel<WM, WN, TM, TN, BK, MODE, false, false>;
+    else if (a.transA && !a.transB) f = gemm_bf16_kernel<WM, WN, TM, TN, BK, MODE, true, false>;
+    else if (!a.transA && a.transB) f = gemm_bf16_kernel<WM, WN, TM, TN, BK, MODE, false, true>;
+    else f = gemm_bf16_kernel<WM, WN, TM, TN, BK, MODE, true, true>;
+    if (smem > 65536)
+        GAT_HIP_TRY(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    const int group_m = tiles_m < 8 ? tiles_m : 8;
+    dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)a.batch);
+    hipLaunchKernelGGL(f, grid, dim3(WM * WN * 64), smem, st, a.A, a.B, (float*)a.C, (int)a.M, (int)a.N, (int)a.K,
+                       a.lda, a.ldb, a.ldc, a.strideA, a.strideB, a.strideC, (float)a.alpha, (float)a.beta, tiles_m,
+                       tiles_n, group_m, vec_ok);
+    GAT_HIP_TRY(hipGetLastError());
+    return ST_OK;
+}
+
+int launch_gemm_mfma_bf16(int dtype, const MfmaCfg& c, const GemmArgs& a, hipStream_t st) {
+    if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return ST_OK;
+    if (a.M > INT32_MAX || a.N > INT32_MAX || a.K > INT32_MAX) return ST_INVALID;
+    if (dtype != DT_BF16 && dtype != DT_BF16X3) return ST_INVALID;
+#define X(p_wm, p_wn, p_tm, p_tn, p_bk)                                                            \
+    if (c.waves_m == p_wm && c.waves_n == p_wn && c.tm == p_tm && c.tn == p_tn && c.bk == p_bk &&  \
+        c.blk == 32)                                                                               \
+        return dtype == DT_BF16 ? launch_bf16_one<p_wm, p_wn, p_tm, p_tn, p_bk, MODE_BF16>(a, st)  \
+                                : launch_bf16_one<p_wm, p_wn, p_tm, p_tn, p_bk, MODE_X3>(a, st);
+    GAT_BF16_CFGS(X)
+#undef X
+    set_error("mfma bf16: configuration not instantiated");
+    return ST_UNSUPPORTED;
+}
+
+// FP32 -> BF16 (round to nearest even), n elements.
+__global__ void f32_to_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = (__bf16)x[i];
+}
+
+int launch_convert_f32_bf16(const float* x, void* y, size_t n, hipStream_t st) {
+    if (!n) return ST_OK;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, st, x, (__bf16*)y, n);
+    GAT_HIP_TRY(hipGetLastError());
+    return ST_OK;
+}
+
+}  // namespace gat

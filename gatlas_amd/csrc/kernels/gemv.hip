@@ -1,12 +1,7 @@
 // GEMV:  y = alpha*op(A)*x + beta*y   (FP32 / FP64), HBM-bound.
 //
 // Native, A row-major (op(A) = A, M x N, row stride lda):
-//   each wavefront owns ROWS consecutive rows and walks them together, every
-//   lane loading one 16-byte vector per row per step (1 KiB contiguous per
-//   row per wave-load, non-temporal since A is streamed once).  x is read
-//   once per step and reused across the ROWS rows.  The per-lane partial
-//   sums are combined by a wavefront butterfly reduction and lane 0 applies
-//   the alpha/beta epilogue.
+//   each wavefront owns ROWS consecutive rows (gemv_rows_wave, gemv_rows.hpp).
 // Native, A transposed (A stored N x M row-major, op(A)[i][j] = A[j*lda + i]):
 //   outputs are contiguous in memory, so a work-item owns 4 (FP32) or 2
 //   (FP64) consecutive outputs and the j range is split into S slices across
@@ -16,72 +11,21 @@
 // Compat (reference geometry, KernelMatvecBuffer.hpp:180-325): work-item g
 //   owns rows g*bh .. g*bh+bh-1, each a j-ascending chain of separately
 //   rounded multiply and add -- the arithmetic of the reference's host check.
-#include "../kernels.hpp"
+#include "gemv_rows.hpp"
 
 namespace gat {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d2v __attribute__((ext_vector_type(2)));
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 template <typename T, typename V, int ROWS, int UNROLL>
 __global__ __launch_bounds__(512) void gemv_rows_kernel(int64_t M, int64_t N, T alpha, const T* __restrict__ A,
                                                          int64_t lda, const T* __restrict__ x, T beta,
                                                          T* __restrict__ y, int vec_ok) {
-    constexpr int W = sizeof(V) / sizeof(T);
-    const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t r0 = wave * ROWS;
     if (r0 >= M) return;
-    T acc[ROWS];
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) acc[r] = 0;
-    int64_t c = 0;
-    if (vec_ok && r0 + ROWS <= M) {
-        const int64_t step = 64 * W;
-        const int64_t nfull = (N / (step * UNROLL)) * (step * UNROLL);
-        for (; c < nfull; c += step * UNROLL) {
-            V av[UNROLL][ROWS], xv[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                const int64_t cc = c + u * step + lane * W;
-                xv[u] = *(const V*)(x + cc);
-#pragma unroll
-                for (int r = 0; r < ROWS; ++r)
-                    av[u][r] = __builtin_nontemporal_load((const V*)(A + (r0 + r) * lda + cc));
-            }
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u)
-#pragma unroll
-                for (int r = 0; r < ROWS; ++r)
-#pragma unroll
-                    for (int e = 0; e < W; ++e) acc[r] += av[u][r][e] * xv[u][e];
-        }
-    }
-    // remainder (and unaligned / partial-row-block cases): scalar columns
-    for (int64_t cc = c + lane; cc < N; cc += 64) {
-        const T xv = x[cc];
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r)
-            if (r0 + r < M) acc[r] += A[(r0 + r) * lda + cc] * xv;
-    }
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) acc[r] = wave_sum<T>(acc[r]);
-    if (lane < ROWS && r0 + lane < M) {
-        T s = acc[0];
-#pragma unroll
-        for (int r = 1; r < ROWS; ++r)
-            if (lane == r) s = acc[r];
-        T v = alpha * s;
-        if (beta != (T)0) v += beta * y[r0 + lane];
-        y[r0 + lane] = v;
-    }
+    gemv_rows_wave<T, V, ROWS, UNROLL>(M, N, alpha, A, lda, x, beta, y, vec_ok, r0, threadIdx.x & 63);
 }
 
 // Transposed: pass 1 partial sums.  Block = 256 work-items covering

@@ -1,0 +1,31 @@
+// Device descriptor of one problem of a grouped GEMM launch.
+#pragma once
+
+#include "../kernels.hpp"
+
+namespace gat {
+
+enum : int {
+    GP_TA = 1,          // op(A) = A^T (A stored K x M)
+    GP_TB = 2,          // op(B) = B^T (B stored N x K)
+    GP_VEC = 4,         // 16-byte aligned operands and leading dimensions
+    GP_SHAPE_SHIFT = 4  // bits 4..7: tile shape (0 = 128x128, 1 = 64x64)
+};
+
+struct GroupedProblemDev {
+    const void* A;
+    const void* B;
+    float* C;
+    int M, N, K;
+    int flags;
+    int64_t lda, ldb, ldc;
+    float alpha, beta;
+};
+
+size_t grouped_smem_bytes(int bf16);
+int launch_grouped_reset(int* counter, hipStream_t st);
+// `work` holds nwork int4 entries {problem, m0, n0, 0}.
+int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* work, int nwork, int* counter,
+                        int grid, hipStream_t st);
+
+}  // namespace gat

@@ -1,0 +1,209 @@
+// One macro tile of the FP32 matrix-core GEMM (v_mfma_f32_32x32x2_f32 /
+// 16x16x4_f32), shared by the per-problem kernel and the persistent grouped
+// kernel.
+//
+//   * macro tile BM x BN = (WM*TM*BLK) x (WN*TN*BLK), WM x WN wavefronts,
+//     each wave owns TM x TN output blocks of BLK x BLK in accumulators;
+//   * K is staged through double-buffered LDS, BK deep.  Both operands are
+//     kept k-contiguous in LDS (A as [m][k], B as [n][k], row stride BK+4
+//     floats) so one ds_read_b128 gives a lane the operands of four MFMAs;
+//     the +4 pad spreads the 32 rows a half-wave reads over distinct banks;
+//   * the next K tile is fetched from HBM into registers while the current
+//     one is multiplied, then written to the other LDS buffer: one barrier
+//     per K step, and no write-after-read hazard on the LDS tiles;
+//   * epilogue C = alpha*acc + beta*C (C untouched when beta == 0), with
+//     each half-wave storing 32 consecutive floats of a row.
+// MFMA k-lane mapping: MFMA s of a group takes, in lane l, the k index
+// kk + 4*(l/32) + s (32x32x2) or kk + 4*(l/16) + s (16x16x4).  A and B use
+// the same mapping, so the product is the full sum over the group's k.
+#pragma once
+
+#include "mfma_common.hpp"
+
+namespace gat {
+
+template <int BLK>
+struct MfmaF32;
+
+template <>
+struct MfmaF32<32> {
+    typedef f16v acc_t;
+    static constexpr int NREG = 16;
+    static constexpr int KGROUP = 8;  // k covered by one b128 read per lane (4 MFMAs)
+    __device__ static __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ int lane_row(int lane) { return lane & 31; }
+    __device__ static __forceinline__ int lane_kq(int lane) { return 4 * (lane >> 5); }
+    __device__ static __forceinline__ int out_row(int lane, int r) { return mfma32_row(lane, r); }
+    __device__ static __forceinline__ int out_col(int lane) { return lane & 31; }
+};
+
+template <>
+struct MfmaF32<16> {
+    typedef f4v acc_t;
+    static constexpr int NREG = 4;
+    static constexpr int KGROUP = 16;
+    __device__ static __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ int lane_row(int lane) { return lane & 15; }
+    __device__ static __forceinline__ int lane_kq(int lane) { return 4 * (lane >> 4); }
+    __device__ static __forceinline__ int out_row(int lane, int r) { return (lane >> 4) * 4 + r; }
+    __device__ static __forceinline__ int out_col(int lane) { return lane & 15; }
+};
+
+// Loads a ROWS x BK tile of a k-contiguous ("direct") or row-contiguous
+// ("transposed") source into registers, and stores it to LDS as [row][k].
+//  direct:     element (r, k) at src[(r0 + r) * ld + k0 + k]
+//  transposed: element (r, k) at src[(k0 + k) * ld + r0 + r]
+// Transposed tiles are walked k-fastest across lanes so that the scalar LDS
+// writes of a wavefront hit consecutive banks.
+template <int ROWS, int BK, int NT, bool TRANS>
+struct TileIO {
+    static constexpr int NV = ROWS * BK / 4 / NT;  // float4 per thread
+    static_assert(NV >= 1 && NV * 4 * NT == ROWS * BK, "tile must split evenly into float4 per thread");
+    static constexpr int LDK = BK + 4;
+
+    __device__ static __forceinline__ void load(f4v (&reg)[NV], const float* __restrict__ src, int64_t ld,
+                                                int r0, int k0, int rows_lim, int k_lim, bool full, int tid) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int idx = q * NT + tid;
+            if (!TRANS) {
+                const int r = idx / (BK / 4), kq = (idx % (BK / 4)) * 4;
+                const int gr = r0 + r, gk = k0 + kq;
+                if (full) {
+                    reg[q] = *(const f4v*)(src + (int64_t)gr * ld + gk);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        reg[q][e] = (gr < rows_lim && gk + e < k_lim) ? src[(int64_t)gr * ld + gk + e] : 0.0f;
+                }
+            } else {
+                const int k = idx % BK, rq = (idx / BK) * 4;
+                const int gk = k0 + k, gr = r0 + rq;
+                if (full) {
+                    reg[q] = *(const f4v*)(src + (int64_t)gk * ld + gr);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        reg[q][e] = (gk < k_lim && gr + e < rows_lim) ? src[(int64_t)gk * ld + gr + e] : 0.0f;
+                }
+            }
+        }
+    }
+
+    __device__ static __forceinline__ void store(const f4v (&reg)[NV], float* lds, int tid) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int idx = q * NT + tid;
+            if (!TRANS) {
+                const int r = idx / (BK / 4), kq = (idx % (BK / 4)) * 4;
+                *(f4v*)(lds + r * LDK + kq) = reg[q];
+            } else {
+                const int k = idx % BK, rq = (idx / BK) * 4;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) lds[(rq + e) * LDK + k] = reg[q][e];
+            }
+        }
+    }
+};
+
+template <int WM, int WN, int TM, int TN, int BK, int BLK>
+struct SgemmShape {
+    static constexpr int NT = WM * WN * 64;
+    static constexpr int BM = WM * TM * BLK, BN = WN * TN * BLK;
+    static constexpr int LDK = BK + 4;
+    static constexpr size_t SMEM = 2 * (size_t)(BM + BN) * LDK * sizeof(float);
+};
+
+// Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C.
+// Must be called by all WM*WN*64 threads of the group; leaves LDS free
+// (ends after a barrier).
+template <int WM, int WN, int TM, int TN, int BK, int BLK, bool TA, bool TB>
+__device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const float* __restrict__ B,
+                                           float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb,
+                                           int64_t ldc, float alpha, float beta, int m0, int n0, bool vec_ok,
+                                           float* smem) {
+    typedef MfmaF32<BLK> Op;
+    typedef SgemmShape<WM, WN, TM, TN, BK, BLK> S;
+    constexpr int NT = S::NT, BM = S::BM, BN = S::BN, LDK = S::LDK;
+    static_assert(BK % Op::KGROUP == 0, "BK must cover whole MFMA k-groups");
+    typedef TileIO<BM, BK, NT, TA> IoA;   // A is direct when op(A) rows are k-contiguous
+    typedef TileIO<BN, BK, NT, !TB> IoB;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const bool full_mn = vec_ok && (m0 + BM <= M) && (n0 + BN <= N);
+
+    typename Op::acc_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < Op::NREG; ++r) acc[i][j][r] = 0.0f;
+
+    f4v ra[IoA::NV], rb[IoB::NV];
+    const int KT = (K + BK - 1) / BK;
+    auto gload = [&](int t) {
+        const int k0 = t * BK;
+        const bool full = full_mn && (k0 + BK <= K);
+        IoA::load(ra, A, lda, m0, k0, M, K, full, tid);
+        IoB::load(rb, B, ldb, n0, k0, N, K, full, tid);
+    };
+
+    gload(0);
+    IoA::store(ra, smem, tid);
+    IoB::store(rb, smem + BM * LDK, tid);
+    __syncthreads();
+
+    const int arow = wm * TM * BLK + Op::lane_row(lane);
+    const int brow = wn * TN * BLK + Op::lane_row(lane);
+    const int kq = Op::lane_kq(lane);
+
+    for (int t = 0; t < KT; ++t) {
+        const float* As = smem + (t & 1) * (BM + BN) * LDK;
+        const float* Bs = As + BM * LDK;
+        if (t + 1 < KT) gload(t + 1);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += Op::KGROUP) {
+            f4v a[TM], b[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a[i] = *(const f4v*)(As + (arow + i * BLK) * LDK + kk + kq);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) b[j] = *(const f4v*)(Bs + (brow + j * BLK) * LDK + kk + kq);
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) acc[i][j] = Op::mma(a[i][s], b[j][s], acc[i][j]);
+        }
+        if (t + 1 < KT) {
+            float* nA = smem + ((t + 1) & 1) * (BM + BN) * LDK;
+            IoA::store(ra, nA, tid);
+            IoB::store(rb, nA + BM * LDK, tid);
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < Op::NREG; ++r) {
+                const int row = m0 + wm * TM * BLK + i * BLK + Op::out_row(lane, r);
+                const int col = n0 + wn * TN * BLK + j * BLK + Op::out_col(lane);
+                if (row < M && col < N) {
+                    float* p = C + (int64_t)row * ldc + col;
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.0f) v += beta * *p;
+                    *p = v;
+                }
+            }
+}
+
+}  // namespace gat

@@ -1,0 +1,90 @@
+// Serving loop: an open-loop trace of GEMM / GEMV requests over
+// device-resident operands, admitted by HBM footprint and continuously
+// batched onto concurrent streams.
+#pragma once
+
+#include <map>
+#include <vector>
+
+#include "../../include/gatlas.h"
+#include "context.hpp"
+#include "kernels/gemv_grouped.hpp"
+#include "kernels/grouped.hpp"
+
+namespace gat {
+
+// First-fit free-list allocator over one device allocation (offsets in
+// bytes, 256-byte granules); adjacent free ranges are merged on release.
+class DeviceArena {
+public:
+    static constexpr size_t kNone = ~(size_t)0;
+    void reset(size_t capacity);
+    size_t alloc(size_t bytes);  // kNone when no free range fits
+    void release(size_t off);
+    size_t in_use() const { return used_; }
+
+private:
+    std::map<size_t, size_t> free_;  // offset -> size
+    std::map<size_t, size_t> live_;  // offset -> size
+    size_t used_ = 0;
+};
+
+struct ServeReq {
+    int kind;  // 0 GEMM, 1 GEMV
+    int ta, tb, shape;
+    int64_t M, N, K, lda, ldb;
+    size_t aoff, boff;     // operand offsets into the resident pool (floats)
+    size_t out_bytes, foot;
+    size_t out_off;        // arena byte offset of C / y
+    double t_arr, t_adm, t_done;
+    bool keep;
+};
+
+class Server {
+public:
+    ~Server();
+    int create(Context& ctx, const gat_serve_cfg& cfg);
+    int run(gat_serve_stats* st);
+    size_t kept() const { return kept_.size(); }
+    int kept_get(size_t i, gat_serve_req* out) const;
+    int check_alone(size_t* mismatches);
+
+private:
+    struct Launch {
+        int stream;
+        hipEvent_t ev;
+        std::vector<int> members;
+    };
+    struct StreamBufs {  // per-stream staging for one batched launch in flight
+        GroupedProblemDev* h_probs = nullptr;
+        int4* h_work = nullptr;
+        GemvProblemDev* h_gv = nullptr;
+        void* d_probs = nullptr;
+        void* d_work = nullptr;
+        void* d_gv = nullptr;
+        int* d_counter = nullptr;
+        bool busy = false;
+    };
+    void draw_trace();
+    float* pool_ptr(size_t off) const { return pool_ + off; }
+    float* out_ptr(const ServeReq& r) const { return (float*)((char*)arena_base_ + r.out_off); }
+    MfmaCfg gemm_cfg(const ServeReq& r) const;
+    int launch_batch(int kind, std::vector<int>& pend, int stream, Launch& L);
+    int launch_one(int id, int stream, Launch& L);
+    hipEvent_t get_event();
+
+    Context* ctx_ = nullptr;
+    gat_serve_cfg cfg_{};
+    std::vector<ServeReq> reqs_;
+    std::vector<int> kept_;
+    float* pool_ = nullptr;
+    size_t pool_elems_ = 0;
+    void* arena_base_ = nullptr;
+    DeviceArena arena_;
+    std::vector<hipStream_t> streams_;
+    std::vector<StreamBufs> bufs_;
+    std::vector<hipEvent_t> events_free_, events_all_;
+    size_t work_cap_ = 0;
+};
+
+}  // namespace gat

@@ -150,39 +150,21 @@ int main(int argc, char* argv[]) {
         return 0;
     }
 
-    const size_t total = k.totalVariations();
-    size_t seedXp = 0;
-    auto enumerate = [&](int step, const std::vector<size_t>& b) {
-        if (variant == VAR_MFMA) {
-            if (step == 0)
-                return getParamsMatmulMfma(k, gemm, M, N, K, tA, tB, std::vector<size_t>(),
-                                           (long)(b.empty() ? seedXp : b[11]));
-            return getParamsMatmulMfma(k, gemm, M, N, K, tA, tB, std::vector<size_t>{b[7], b[8], b[9], b[10]}, -1);
-        }
-        if (step == 0)
-            return getParamsMatmul(k, app.maxWorkGroupSize(), vl, maxBH, maxGS, gemm, M, N, K, tA, tB, -1, -1,
-                                   (long)(b.empty() ? seedXp : b[11]));
-        return getParamsMatmul(k, app.maxWorkGroupSize(), vl, maxBH, maxGS, gemm, M, N, K, tA, tB, (long)b[7],
-                               (long)b[9], -1);
-    };
-    auto bump = [&](std::vector<size_t>& b) {
-        if (!b.empty()) {
-            b[11] = (b[11] + 1) % total;
-            return true;
-        }
-        return ++seedXp < total;
-    };
-    const std::vector<size_t> best = emSearch(k, bench, journal, o, {7, 8, 9, 10, 11}, enumerate, bump);
+    SearchSpec spec;
+    spec.maxWorkGroupSize = app.maxWorkGroupSize();
+    spec.vl = vl;
+    spec.maxBlockHeight = maxBH;
+    spec.maxGroupSize = maxGS;
+    spec.generalized = gemm;
+    spec.tA = tA;
+    spec.tB = tB;
+    spec.M = M;
+    spec.N = N;
+    spec.K = K;
+    const std::vector<size_t> best = emSearchKernel(k, bench, journal, o, spec);
     if (best.empty()) {
         std::cerr << "***DONE***" << std::endl;
         return 1;
-    }
-    if (trials > 1) {
-        std::vector<std::vector<size_t>> pargs{best};
-        std::vector<bool> ok(1, true);
-        std::vector<double> avg(1, 0.0);
-        journal.loadMemo();
-        mainLoop(k, bench, journal, pargs, ok, avg, o);
     }
     std::cout << "***DONE***" << std::endl;
     return 0;

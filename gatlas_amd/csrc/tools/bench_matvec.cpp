@@ -111,30 +111,19 @@ int main(int argc, char* argv[]) {
         std::cout << "***DONE***" << std::endl;
         return 0;
     }
-    const size_t total = k.totalVariations();
-    size_t seedXp = 0;
-    auto enumerate = [&](int step, const std::vector<size_t>& b) {
-        if (step == 0) return params(-1, -1, (long)(b.empty() ? seedXp : b[8]));
-        return params((long)b[5], (long)b[6], -1);
-    };
-    auto bump = [&](std::vector<size_t>& b) {
-        if (!b.empty()) {
-            b[8] = (b[8] + 1) % total;
-            return true;
-        }
-        return ++seedXp < total;
-    };
-    const std::vector<size_t> best = emSearch(k, bench, journal, o, {5, 6, 8}, enumerate, bump);
+    SearchSpec spec;
+    spec.maxWorkGroupSize = app.maxWorkGroupSize();
+    spec.vl = vl;
+    spec.maxBlockHeight = maxBH;
+    spec.maxGroupSize = maxGS;
+    spec.generalized = gemv;
+    spec.tA = tA;
+    spec.M = M;
+    spec.N = N;
+    const std::vector<size_t> best = emSearchKernel(k, bench, journal, o, spec);
     if (best.empty()) {
         std::cerr << "***DONE***" << std::endl;
         return 1;
-    }
-    if (trials > 1) {
-        std::vector<std::vector<size_t>> pargs{best};
-        std::vector<bool> ok(1, true);
-        std::vector<double> avg(1, 0.0);
-        journal.loadMemo();
-        mainLoop(k, bench, journal, pargs, ok, avg, o);
     }
     std::cout << "***DONE***" << std::endl;
     return 0;

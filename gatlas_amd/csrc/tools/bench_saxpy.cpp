@@ -106,27 +106,17 @@ int main(int argc, char* argv[]) {
         std::cout << "***DONE***" << std::endl;
         return 0;
     }
-    // step 0: group shape free, blocking and extra parameter fixed;
-    // step 1: group fixed, blocking and extra parameter free.
-    const size_t initBW = ((size_t)N == vl) ? 0 : vl;
-    auto enumerate = [&](int step, const std::vector<size_t>& b) {
-        if (step == 0) {
-            if (b.empty()) return params(-1, -1, 1, (long)initBW, 0);
-            return params(-1, -1, (long)b[5], (long)b[6], (long)b[8]);
-        }
-        return params((long)b[3], (long)b[4], -1, -1, -1);
-    };
-    const std::vector<size_t> best = emSearch(k, bench, journal, o, {3, 4, 5, 6, 8}, enumerate, nullptr);
+    SearchSpec spec;
+    spec.maxWorkGroupSize = app.maxWorkGroupSize();
+    spec.vl = vl;
+    spec.maxBlockHeight = maxBH;
+    spec.maxGroupSize = maxGS;
+    spec.M = M;
+    spec.N = N;
+    const std::vector<size_t> best = emSearchKernel(k, bench, journal, o, spec);
     if (best.empty()) {
         std::cerr << "***DONE***" << std::endl;
         return 1;
-    }
-    if (trials > 1) {
-        std::vector<std::vector<size_t>> pargs{best};
-        std::vector<bool> ok(1, true);
-        std::vector<double> avg(1, 0.0);
-        journal.loadMemo();
-        mainLoop(k, bench, journal, pargs, ok, avg, o);
     }
     std::cout << "***DONE***" << std::endl;
     return 0;

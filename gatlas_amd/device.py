@@ -158,6 +158,56 @@ class Context:
         check(self.lib.gat_axpy_compat(self.h, dtype_code(z.dtype), M, N, alpha, x.ptr, y.ptr, z.ptr, gH, gW, bh,
                                        bw, vl, packed), "gat_axpy_compat")
 
+    def convert_bf16(self, src, dst):
+        """FP32 device array -> BF16 (stored in a uint16 device array)."""
+        check(self.lib.gat_convert_f32_bf16(self.h, src.ptr, dst.ptr, src.size), "convert_bf16")
+
+    def convert_bf16_n(self, src, dst, n):
+        """The first n elements of src -> BF16."""
+        assert n <= src.size and n <= dst.size
+        check(self.lib.gat_convert_f32_bf16(self.h, src.ptr, dst.ptr, n), "convert_bf16")
+
+    def quantize_fp8(self, src, rows, cols, ld, trans, out, scales):
+        check(self.lib.gat_quantize_fp8(self.h, src.ptr, rows, cols, ld, int(trans), out.ptr, scales.ptr),
+              "quantize_fp8")
+
+    def gemm_fp8(self, A8, sA, B8, sB, C, M, N, K, alpha=1.0, beta=0.0, ldc=None):
+        check(self.lib.gat_gemm_fp8(self.h, M, N, K, alpha, A8.ptr, sA.ptr, B8.ptr, sB.ptr, beta, C.ptr,
+                                    N if ldc is None else ldc), "gat_gemm_fp8")
+
+    # ---- grouped GEMM ----
+    @staticmethod
+    def gemm_descs(problems):
+        """problems: sequence of dicts with transA, transB, M, N, K, alpha, A, B,
+        beta, C (DeviceArrays) and optional lda, ldb, ldc."""
+        arr = (_capi.GemmDesc * max(1, len(problems)))()
+        for i, p in enumerate(problems):
+            ta, tb = int(p.get("transA", 0)), int(p.get("transB", 0))
+            M, N, K = p["M"], p["N"], p["K"]
+            arr[i] = _capi.GemmDesc(ta, tb, M, N, K, p.get("alpha", 1.0), p["A"].ptr,
+                                    p.get("lda", M if ta else K), p["B"].ptr, p.get("ldb", K if tb else N),
+                                    p.get("beta", 0.0), p["C"].ptr, p.get("ldc", N))
+        return arr
+
+    def gemv_grouped(self, problems):
+        """problems: dicts with M, N, A, x, y (FP32 DeviceArrays) and optional
+        alpha, beta, lda."""
+        arr = (_capi.GemvDesc * max(1, len(problems)))()
+        for i, p in enumerate(problems):
+            arr[i] = _capi.GemvDesc(p["M"], p["N"], p.get("alpha", 1.0), p["A"].ptr, p.get("lda", p["N"]),
+                                    p["x"].ptr, p.get("beta", 0.0), p["y"].ptr)
+        check(self.lib.gat_gemv_grouped(self.h, arr, len(problems)), "gat_gemv_grouped")
+
+    def grouped(self, problems, dtype=_capi.GAT_F32, use_graph=True):
+        return Grouped(self, problems, dtype, use_graph)
+
+    def grouped_tune(self, problems, dtype=_capi.GAT_F32, reps=2):
+        descs = self.gemm_descs(problems)
+        n = self.lib.gat_grouped_tune(self.h, dtype, descs, len(problems), reps)
+        if n < 0:
+            raise _capi.GatError("grouped_tune: " + _capi.last_error())
+        return n
+
     @staticmethod
     def tile_cfgs(dtype):
         lib = load()
@@ -167,3 +217,90 @@ class Context:
             check(lib.gat_tile_cfg_get(dtype, i, ctypes.byref(c)), "tile_cfg_get")
             out.append(c.as_tuple())
         return out
+
+
+class Grouped:
+    """A persistent grouped-GEMM plan (optionally a captured HIP graph)."""
+
+    def __init__(self, ctx, problems, dtype, use_graph):
+        self.ctx = ctx
+        self.n = len(problems)
+        self._descs = ctx.gemm_descs(problems)
+        self.h = ctx.lib.gat_grouped_create(ctx.h, dtype, self._descs, self.n, int(use_graph))
+        if not self.h:
+            raise _capi.GatError("gat_grouped_create: " + _capi.last_error())
+
+    def run(self):
+        check(self.ctx.lib.gat_grouped_run(self.h), "gat_grouped_run")
+
+    def info(self):
+        t, f, g = ctypes.c_size_t(), ctypes.c_double(), ctypes.c_int()
+        check(self.ctx.lib.gat_grouped_info(self.h, ctypes.byref(t), ctypes.byref(f), ctypes.byref(g)), "info")
+        return {"tiles": t.value, "flops": f.value, "grid": g.value}
+
+    def shapes(self):
+        return [self.ctx.lib.gat_grouped_shape(self.h, i) for i in range(self.n)]
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.gat_grouped_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop of the C ABI (gat_server_*): a Poisson trace of GEMM /
+    GEMV requests over resident operands, admitted by HBM footprint."""
+
+    def __init__(self, ctx, **kw):
+        self.ctx = ctx
+        cfg = _capi.ServeCfg()
+        ctx.lib.gat_serve_cfg_default(ctypes.byref(cfg))
+        for k, v in kw.items():
+            if not hasattr(cfg, k):
+                raise KeyError(k)
+            setattr(cfg, k, v)
+        self.cfg = cfg
+        self.h = ctx.lib.gat_server_create(ctx.h, ctypes.byref(cfg))
+        if not self.h:
+            raise _capi.GatError("gat_server_create: " + _capi.last_error())
+
+    def run(self):
+        st = _capi.ServeStats()
+        check(self.ctx.lib.gat_server_run(self.h, ctypes.byref(st)), "gat_server_run")
+        return st.as_dict()
+
+    def kept(self):
+        out = []
+        for i in range(self.ctx.lib.gat_server_kept(self.h)):
+            r = _capi.ServeReq()
+            check(self.ctx.lib.gat_server_kept_get(self.h, i, ctypes.byref(r)), "kept_get")
+            out.append({n: getattr(r, n) for n, _ in r._fields_})
+        return out
+
+    def check_alone(self):
+        m = ctypes.c_size_t()
+        check(self.ctx.lib.gat_server_check_alone(self.h, ctypes.byref(m)), "check_alone")
+        return m.value
+
+    def read(self, ptr, count):
+        """Copy `count` FP32 values from a device address the server owns."""
+        out = np.empty(count, dtype=np.float32)
+        check(self.ctx.lib.gat_memcpy_d2h(self.ctx.h, out.ctypes.data, ptr, out.nbytes), "d2h")
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.gat_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

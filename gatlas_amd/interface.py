@@ -117,6 +117,18 @@ class Kernel:
         return [list(out[i * self.nparams:(i + 1) * self.nparams]) for i in range(n)]
 
 
+    def em_search(self, journal_path, M, N, K=0, transA=False, transB=False, generalized=False, vl=4,
+                  max_workgroup=1024, max_block_height=10, max_group_size=10, ntrials=1, paranoid=False):
+        """EM search (benchMatmul.cpp:536-650); replays the journal, runs the
+        device only for candidates without a record.  Returns the best
+        parameter vector or None."""
+        out = (ctypes.c_size_t * 16)()
+        n = self.lib.gat_search_em(self.h, journal_path.encode(), max_workgroup, vl, max_block_height,
+                                   max_group_size, int(generalized), M, N, K, int(transA), int(transB), ntrials,
+                                   int(paranoid), out, 16)
+        return list(out[:n]) if n > 0 else None
+
+
 class Journal:
     MISSING, BUILD_IN_PROGRESS, BUILD_OK, RUN_IN_PROGRESS, RUN_OK = 0, -1, -2, -3, -4
 

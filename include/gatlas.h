@@ -109,8 +109,8 @@ int gat_scale_columns(gat_ctx *ctx, float *dptr, size_t rows, size_t cols, uint6
  * dtype GAT_F32 / GAT_F64: A, B, C of that type.
  * dtype GAT_BF16: A, B are bf16 arrays, C is FP32.
  * dtype GAT_BF16X3: A, B, C FP32 (split inside the library).
- * dtype GAT_FP8: A, B are e4m3 bytes of op(A) (M x K row-major) and op(B)
- *   stored as N x K row-major, scales via gat_gemm_fp8. */
+ * dtype GAT_FP8: A, B, C FP32; A and B are block-quantised to e4m3 in the
+ *   context workspace, then multiplied (see gat_gemm_fp8 for resident e4m3). */
 int gat_gemm(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, size_t N, size_t K, double alpha,
              const void *A, size_t lda, const void *B, size_t ldb, double beta, void *C, size_t ldc);
 
@@ -142,6 +142,19 @@ int gat_gemm_compat(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, s
                     double alpha, const void *A, const void *B, double beta, void *C, size_t gs, size_t bh,
                     size_t vl, size_t packed);
 
+/* FP32 -> BF16 (round to nearest even), n elements (BF16-input GEMM operands). */
+int gat_convert_f32_bf16(gat_ctx *ctx, const float *src, void *dst, size_t n);
+/* FP8 e4m3 block quantisation of a rows x cols operand whose element (r, c) is
+ * trans ? src[c*ld + r] : src[r*ld + c]: writes rows x cols bytes row-major and
+ * ceil(rows/128) x ceil(cols/128) FP32 block scales (powers of two >= amax/448). */
+int gat_quantize_fp8(gat_ctx *ctx, const float *src, size_t rows, size_t cols, size_t ld, int trans, void *out,
+                     float *scales);
+/* C = alpha * (A8 .* sA) (B8 .* sB)^T + beta*C with A8 = op(A) as M x K e4m3 and
+ * B8 = op(B)^T as N x K e4m3 (both from gat_quantize_fp8), FP32 accumulate.
+ * gat_gemm with dtype GAT_FP8 takes FP32 operands and quantises them first. */
+int gat_gemm_fp8(gat_ctx *ctx, size_t M, size_t N, size_t K, double alpha, const void *A8, const float *scaleA,
+                 const void *B8, const float *scaleB, double beta, float *C, size_t ldc);
+
 /* y = alpha*op(A)*x + beta*y; op(A) is M x N (A stored N x M when transA). */
 int gat_gemv(gat_ctx *ctx, int dtype, int transA, size_t M, size_t N, double alpha, const void *A, size_t lda,
              const void *x, double beta, void *y);
@@ -156,6 +169,120 @@ int gat_axpy(gat_ctx *ctx, int dtype, size_t n, double alpha, const void *x, con
 /* Reference geometry: M x N scalars, 1-D when N == vl (KernelBaseSaxpy.cpp:289-301). */
 int gat_axpy_compat(gat_ctx *ctx, int dtype, size_t M, size_t N, double alpha, const void *x, const void *y,
                     void *z, size_t gH, size_t gW, size_t bh, size_t bw, size_t vl, size_t packed);
+
+/* ---------------------------------------------------------------------- */
+/* Grouped GEMM (no reference counterpart; the reference's only batching   */
+/* is packedCalc, same-shape problems in one kernel): many independent     */
+/* problems of any shapes/transposes in ONE persistent launch, optionally  */
+/* captured in a HIP graph.  dtype GAT_F32 (FP32 operands) or GAT_BF16     */
+/* (bf16 operands); C is FP32.                                             */
+/* ---------------------------------------------------------------------- */
+typedef struct gat_gemm_desc {
+    int transA, transB;
+    size_t M, N, K;
+    double alpha;
+    const void *A;
+    size_t lda;
+    const void *B;
+    size_t ldb;
+    double beta;
+    void *C;
+    size_t ldc;
+} gat_gemm_desc;
+typedef struct gat_grouped gat_grouped;
+/* Builds the problem table and the cost-ordered tile list (device-resident;
+ * tile shape per problem from the context's tuned size buckets). */
+gat_grouped *gat_grouped_create(gat_ctx *ctx, int dtype, const gat_gemm_desc *descs, size_t count,
+                                int use_graph);
+/* One persistent launch (a graph replay when created with use_graph). */
+int gat_grouped_run(gat_grouped *g);
+int gat_grouped_info(gat_grouped *g, size_t *tiles, double *flops, int *grid);
+/* Tile shape chosen for problem i: 0 = 128x128, 1 = 64x64. */
+int gat_grouped_shape(gat_grouped *g, size_t i);
+void gat_grouped_destroy(gat_grouped *g);
+/* create + run + synchronise + destroy. */
+int gat_gemm_grouped(gat_ctx *ctx, int dtype, const gat_gemm_desc *descs, size_t count);
+/* Times both tile shapes on one problem per size bucket and installs the faster
+ * in the context's tuned table; returns the number of buckets tuned. */
+int gat_grouped_tune(gat_ctx *ctx, int dtype, const gat_gemm_desc *descs, size_t count, int reps);
+
+/* Grouped GEMV (FP32, A row-major M x N): y = alpha*A*x + beta*y for many
+ * problems in one launch; each result is bit-identical to gat_gemv's. */
+typedef struct gat_gemv_desc {
+    size_t M, N;
+    double alpha;
+    const float *A;
+    size_t lda;
+    const float *x;
+    double beta;
+    float *y;
+} gat_gemv_desc;
+int gat_gemv_grouped(gat_ctx *ctx, const gat_gemv_desc *descs, size_t count);
+
+/* ---------------------------------------------------------------------- */
+/* Serving loop (no reference counterpart): a synthetic open-loop trace of */
+/* GEMM and GEMV requests (Poisson arrivals) over device-resident          */
+/* operands, admitted by HBM footprint and continuously batched onto       */
+/* concurrent streams; reports latency percentiles and throughput.         */
+/* ---------------------------------------------------------------------- */
+#define GAT_SERVE_BATCHED 0 /* admitted requests grouped into persistent launches */
+#define GAT_SERVE_LOOP 1    /* one launch per request (the per-problem loop)      */
+
+typedef struct gat_serve_cfg {
+    double rate;                  /* mean arrivals per second (Poisson)            */
+    double duration;              /* seconds of arrivals                           */
+    double gemm_fraction;         /* share of GEMM requests; the rest are GEMV     */
+    size_t hbm_budget;            /* bytes of admitted footprint at once           */
+    int mode;                     /* GAT_SERVE_BATCHED or GAT_SERVE_LOOP           */
+    int streams;                  /* device streams launches overlap on            */
+    int max_batch;                /* problems per batched launch                   */
+    int max_inflight;             /* admitted, unfinished requests at once         */
+    int gemm_min, gemm_max;       /* GEMM M, N, K log-uniform, multiples of 16     */
+    int gemv_min, gemv_max;       /* GEMV M, N log-uniform                         */
+    int keep;                     /* keep every keep-th result (0: none) ...       */
+    int keep_max;                 /* ... up to keep_max results                    */
+    unsigned long long seed;
+} gat_serve_cfg;
+
+typedef struct gat_serve_stats {
+    size_t requests, gemm_requests, gemv_requests, launches, kept;
+    double wall_s;                /* first arrival to last completion              */
+    double flops, bytes;          /* algorithmic flops, operand+result bytes       */
+    double tflops, req_per_s;
+    double p50_ms, p99_ms, max_ms, p50_gemm_ms, p99_gemm_ms, p50_gemv_ms, p99_gemv_ms;
+    double mean_queue_ms;         /* arrival to admission                          */
+    double mean_batch;            /* requests per launch                           */
+    double peak_footprint;        /* bytes admitted at once, maximum               */
+} gat_serve_stats;
+
+typedef struct gat_serve_req {
+    int kind;                     /* 0 GEMM, 1 GEMV                                */
+    int transA, transB, shape;    /* shape: GEMM tile shape (gat_grouped_shape)    */
+    size_t M, N, K;               /* GEMV: K = 0                                   */
+    const float *A;
+    size_t lda;
+    const float *B;               /* GEMV: x                                       */
+    size_t ldb;
+    float *C;                     /* GEMV: y                                       */
+    size_t ldc;
+    double latency_ms;
+    size_t id;                    /* index of the request in the trace             */
+} gat_serve_req;
+
+typedef struct gat_server gat_server;
+void gat_serve_cfg_default(gat_serve_cfg *cfg);
+/* Allocates the resident operand pool and the output arena (hbm_budget
+ * bytes), draws the trace; nothing is launched yet. */
+gat_server *gat_server_create(gat_ctx *ctx, const gat_serve_cfg *cfg);
+/* Replays the trace in real time; returns when every request finished. */
+int gat_server_run(gat_server *s, gat_serve_stats *stats);
+/* Kept results of the last run (device pointers valid until destroy). */
+size_t gat_server_kept(gat_server *s);
+int gat_server_kept_get(gat_server *s, size_t i, gat_serve_req *out);
+/* Recomputes every kept request alone (single launch, same kernel and tile
+ * shape) and counts results that differ in any bit. */
+int gat_server_check_alone(gat_server *s, size_t *mismatches);
+void gat_server_destroy(gat_server *s);
 
 /* ---------------------------------------------------------------------- */
 /* Parameter interface: the KernelInterface boundary                      */
@@ -238,6 +365,15 @@ int gat_mark_bench_avg(size_t topN, int *ok, const double *avg, size_t n);
 int gat_search_params(gat_kernel *k, size_t max_workgroup, size_t vl, size_t max_block_height,
                       size_t max_group_size, int generalized, size_t M, size_t N, size_t K, int transA, int transB,
                       long group_size, long block_height, long extra_param, size_t *out, size_t max_out);
+
+/* The two-step EM search of benchMatmul.cpp:536-650 / benchMatvec.cpp:521-640 /
+ * benchSaxpy.cpp:498-600 for kernel k, journaled to journal_path (trials already
+ * in the journal are replayed, bad keys skipped).  Without a context (k created
+ * with ctx NULL) only the journal is replayed.  Writes the best parameter vector;
+ * returns its length, or <0. */
+int gat_search_em(gat_kernel *k, const char *journal_path, size_t max_workgroup, size_t vl,
+                  size_t max_block_height, size_t max_group_size, int generalized, size_t M, size_t N, size_t K,
+                  int transA, int transB, size_t ntrials, int paranoid, size_t *best, size_t nbest);
 
 /* Autotuner (no reference counterpart; replaces the exhaustive OpenCL search for
  * the matrix-core kernels): EM search over the instantiated tile table for one

@@ -31,6 +31,33 @@ def splitmix64_stream(seed, n):
         return _mix64(x)
 
 
+def e4m3_table():
+    """Value of every OCP FP8 e4m3 byte (bias 7, no infinities, 0x7F/0xFF NaN):
+    exponent 0 -> m * 2^-9, else (1 + m/8) * 2^(e-7)."""
+    t = np.zeros(256, dtype=np.float64)
+    for b in range(256):
+        e, m = (b >> 3) & 15, b & 7
+        v = m * 2.0 ** -9 if e == 0 else (1 + m / 8.0) * 2.0 ** (e - 7)
+        if (b & 0x7F) == 0x7F:
+            v = np.nan
+        t[b] = -v if b & 0x80 else v
+    return t
+
+
+def bf16_bits_to_f32(u16):
+    """BF16 bit patterns (uint16) -> float32 values."""
+    return (np.asarray(u16, dtype=np.uint32) << np.uint32(16)).view(np.float32)
+
+
+def dequant_fp8(q, scales, rows, cols):
+    """q (rows x cols e4m3 bytes) times its 128 x 128 block scales, in FP64."""
+    t = e4m3_table()
+    v = t[q.reshape(rows, cols)]
+    cb = (cols + 127) // 128
+    s = np.asarray(scales, dtype=np.float64).reshape(-1, cb)
+    return v * np.repeat(np.repeat(s, 128, axis=0)[:rows], 128, axis=1)[:, :cols]
+
+
 def op(M_, rows, cols, trans):
     """op(X) as a (rows x cols) view of the flat row-major storage."""
     return M_.reshape(cols, rows).T if trans else M_.reshape(rows, cols)

@@ -45,3 +45,33 @@ def test_tile_cfg_tables_are_static():
         assert lib.gat_tile_cfg_get(dt, 0, ctypes.byref(c)) == 0
         assert c.waves_m >= 1 and c.blk in (16, 32)
         assert lib.gat_tile_cfg_get(dt, n, ctypes.byref(c)) != 0
+
+
+def test_ctypes_structs_match_c_layout(tmp_path):
+    """Every struct of the header has the same size and field offsets in C
+    (gcc) as in the ctypes mirror."""
+    import ctypes
+    import shutil
+    import subprocess
+    import pytest
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {"gat_tile_cfg": _capi.TileCfg, "gat_gemm_desc": _capi.GemmDesc, "gat_gemv_desc": _capi.GemvDesc,
+               "gat_serve_cfg": _capi.ServeCfg, "gat_serve_stats": _capi.ServeStats,
+               "gat_serve_req": _capi.ServeReq}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gatlas.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
+        for f, _ in py._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = [int(v) for v in subprocess.check_output([str(exe)]).split()]
+    want = []
+    for py in structs.values():
+        want.append(ctypes.sizeof(py))
+        want += [getattr(py, f).offset for f, _ in py._fields_]
+    assert got == want

@@ -1,0 +1,192 @@
+"""Persistent grouped GEMM on the GPU.
+
+  * every problem of a mixed group (ragged sizes, all transpose pairs, general
+    alpha/beta, empty problems) against the FP64 oracle: componentwise
+    error <= 1e-5 (FP32) and <= 1e-5 against the bf16-rounded operands (BF16);
+  * each problem's result is bit-identical to a single launch of the same
+    tile configuration (a problem computes the same alone as in the group);
+  * a captured graph replayed several times, with eager work between the
+    replays, gives the eager result every time;
+  * 512 problems of the grouped workload (M, N, K log-uniform in [64, 4096],
+    multiples of 16, random transposes): bit-identical to the per-problem
+    launch loop, a row sample of every problem against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from gatlas_amd import _capi
+from oracle import oracle as ora
+from tests.helpers import bf16_bits_to_f32, uniform_host
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+SHAPE_CFG = {_capi.GAT_F32: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 32, 32)],
+             _capi.GAT_BF16: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 64, 32)]}
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _bf16_round(x):
+    import torch
+    return torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+
+
+def _make_group(ctx, dims, seed, dtype=_capi.GAT_F32, general=True):
+    """Host operands + device problems for a list of (M, N, K, tA, tB)."""
+    host, probs = [], []
+    for i, (M, N, K, ta, tb) in enumerate(dims):
+        A = uniform_host(M * K, seed, 3 * i)
+        B = uniform_host(K * N, seed, 3 * i + 1)
+        C0 = uniform_host(M * N, seed, 3 * i + 2)
+        alpha, beta = (1.5, -0.5) if general else (1.0, 0.0)
+        if dtype == _capi.GAT_BF16:
+            a16, b16 = _bf16_round(A), _bf16_round(B)
+            dA, dB = ctx.to_device(a16), ctx.to_device(b16)
+            A, B = bf16_bits_to_f32(a16), bf16_bits_to_f32(b16)
+        else:
+            dA, dB = ctx.to_device(A), ctx.to_device(B)
+        dC = ctx.to_device(C0)
+        host.append((A, B, C0, alpha, beta))
+        probs.append(dict(transA=ta, transB=tb, M=M, N=N, K=K, alpha=alpha, beta=beta, A=dA, B=dB, C=dC))
+    return host, probs
+
+
+def _check_oracle(dims, host, probs, rows_per=None):
+    worst = 0.0
+    for (M, N, K, ta, tb), (A, B, C0, alpha, beta), p in zip(dims, host, probs):
+        got = p["C"].download().reshape(M, N)
+        if M == 0 or N == 0:
+            continue
+        rows = None if rows_per is None else np.unique(np.linspace(0, M - 1, min(M, rows_per)).astype(np.int64))
+        ref, mag = ora.gemm64(A, B, M, N, K, bool(ta), bool(tb), alpha, beta, C0, rows=rows)
+        g = got if rows is None else got[rows]
+        cw, _, _ = ora.error_metrics(np.ascontiguousarray(g), ref, mag)
+        assert cw <= TOL, ((M, N, K, ta, tb), cw)
+        worst = max(worst, cw)
+    return worst
+
+
+def _ragged_dims(rng, n, lo, hi):
+    dims = []
+    for _ in range(n):
+        M, N, K = (int(v) for v in rng.integers(lo, hi, 3))
+        dims.append((M, N, K, int(rng.integers(0, 2)), int(rng.integers(0, 2))))
+    return dims
+
+
+@pytest.mark.parametrize("dtype", [_capi.GAT_F32, _capi.GAT_BF16], ids=["f32", "bf16"])
+def test_grouped_mixed_against_oracle(ctx, dtype):
+    rng = np.random.default_rng(7)
+    dims = _ragged_dims(rng, 24, 1, 300)
+    dims += [(0, 5, 7, 0, 0), (5, 0, 7, 1, 0), (5, 7, 0, 0, 1), (256, 256, 256, 1, 1), (300, 260, 33, 0, 1)]
+    _say("grouped mixed", dtype, len(dims))
+    host, probs = _make_group(ctx, dims, 11, dtype)
+    g = ctx.grouped(probs, dtype, use_graph=False)
+    g.run()
+    ctx.sync()
+    worst = _check_oracle(dims, host, probs)
+    print("[grouped] dtype %d worst componentwise err %.3e, %s" % (dtype, worst, g.info()))
+
+
+@pytest.mark.parametrize("dtype", [_capi.GAT_F32, _capi.GAT_BF16], ids=["f32", "bf16"])
+def test_grouped_equals_single_launch(ctx, dtype):
+    rng = np.random.default_rng(8)
+    dims = _ragged_dims(rng, 12, 16, 520)
+    _say("grouped vs single", dtype)
+    host, probs = _make_group(ctx, dims, 12, dtype)
+    g = ctx.grouped(probs, dtype, use_graph=True)
+    g.run()
+    ctx.sync()
+    shapes = g.shapes()
+    for (M, N, K, ta, tb), (A, B, C0, alpha, beta), p, s in zip(dims, host, probs, shapes):
+        grouped = p["C"].download()
+        p["C"].upload(C0)
+        ctx.gemm(p["A"], p["B"], p["C"], M, N, K, bool(ta), bool(tb), alpha, beta, dtype=dtype,
+                 cfg=SHAPE_CFG[dtype][s])
+        ctx.sync()
+        single = p["C"].download()
+        assert np.array_equal(grouped.view(np.uint32), single.view(np.uint32)), (M, N, K, ta, tb, s)
+
+
+def test_grouped_graph_replay(ctx):
+    rng = np.random.default_rng(9)
+    dims = _ragged_dims(rng, 20, 8, 400)
+    _say("grouped graph replay")
+    host, probs = _make_group(ctx, dims, 13, general=False)  # beta = 0: replays are idempotent
+    eager = ctx.grouped(probs, _capi.GAT_F32, use_graph=False)
+    eager.run()
+    ctx.sync()
+    want = [p["C"].download() for p in probs]
+    graph = ctx.grouped(probs, _capi.GAT_F32, use_graph=True)
+    junk = ctx.empty(1 << 20)
+    for rep in range(4):
+        for p in probs:  # eager work between replays: clobber every output
+            ctx.fill_uniform(p["C"], 99, rep)
+        ctx.fill_uniform(junk, 98, rep)
+        ctx.sync()
+        graph.run()
+        ctx.sync()
+        for w, p in zip(want, probs):
+            assert np.array_equal(p["C"].download().view(np.uint32), w.view(np.uint32)), rep
+    _check_oracle(dims, host, probs)
+
+
+def _workload_dims(rng, n):
+    """M, N, K log-uniform in [64, 4096], rounded to multiples of 16."""
+    v = np.exp(rng.uniform(np.log(64), np.log(4096), (n, 3)))
+    v = np.clip(np.round(v / 16) * 16, 64, 4096).astype(int)
+    t = rng.integers(0, 2, (n, 2))
+    return [(int(a), int(b), int(c), int(x), int(y)) for (a, b, c), (x, y) in zip(v, t)]
+
+
+@pytest.mark.parametrize("dtype", [_capi.GAT_F32, _capi.GAT_BF16], ids=["f32", "bf16"])
+def test_grouped_512_workload(ctx, dtype):
+    rng = np.random.default_rng(2024)
+    dims = _workload_dims(rng, 512)
+    _say("grouped 512", dtype, "flops %.3e" % sum(2.0 * m * n * k for m, n, k, _, _ in dims))
+    probs, outs = [], []
+    esz = 2 if dtype == _capi.GAT_BF16 else 4
+    for i, (M, N, K, ta, tb) in enumerate(dims):
+        if dtype == _capi.GAT_BF16:
+            fA, fB = ctx.empty(M * K), ctx.empty(K * N)
+            ctx.fill_uniform(fA, 21, 2 * i)
+            ctx.fill_uniform(fB, 21, 2 * i + 1)
+            dA, dB = ctx.empty(M * K, np.uint16), ctx.empty(K * N, np.uint16)
+            ctx.convert_bf16(fA, dA)
+            ctx.convert_bf16(fB, dB)
+            fA.free()
+            fB.free()
+        else:
+            dA, dB = ctx.empty(M * K), ctx.empty(K * N)
+            ctx.fill_uniform(dA, 21, 2 * i)
+            ctx.fill_uniform(dB, 21, 2 * i + 1)
+        dC = ctx.empty(M * N)
+        probs.append(dict(transA=ta, transB=tb, M=M, N=N, K=K, alpha=1.0, beta=0.0, A=dA, B=dB, C=dC))
+    assert esz in (2, 4)
+    ctx.grouped_tune(probs, dtype)
+    g = ctx.grouped(probs, dtype, use_graph=True)
+    g.run()
+    ctx.sync()
+    shapes = g.shapes()
+    got = [p["C"].download() for p in probs]
+    # per-problem launch loop with the same tile choice: bit-identical
+    ref = ctx.empty(max(m * n for m, n, _, _, _ in dims))
+    for i, ((M, N, K, ta, tb), p, s) in enumerate(zip(dims, probs, shapes)):
+        ctx.gemm(p["A"], p["B"], ref, M, N, K, bool(ta), bool(tb), 1.0, 0.0, dtype=dtype, cfg=SHAPE_CFG[dtype][s])
+        ctx.sync()
+        single = ref.download()[:M * N]
+        assert np.array_equal(single.view(np.uint32), got[i].view(np.uint32)), (i, M, N, K, ta, tb, s)
+    # a row sample of every 16th problem against the FP64 oracle
+    for i in range(0, 512, 16):
+        M, N, K, ta, tb = dims[i]
+        A = uniform_host(M * K, 21, 2 * i)
+        B = uniform_host(K * N, 21, 2 * i + 1)
+        if dtype == _capi.GAT_BF16:
+            A, B = bf16_bits_to_f32(_bf16_round(A)), bf16_bits_to_f32(_bf16_round(B))
+        rows = np.unique(np.linspace(0, M - 1, 4).astype(np.int64))
+        r64, mag = ora.gemm64(A, B, M, N, K, bool(ta), bool(tb), 1.0, 0.0, None, rows=rows)
+        cw, _, _ = ora.error_metrics(np.ascontiguousarray(got[i].reshape(M, N)[rows]), r64, mag)
+        assert cw <= TOL, (i, dims[i], cw)
+    print("[grouped512] dtype %d %s shapes(128x128)=%d" % (dtype, g.info(), shapes.count(0)))

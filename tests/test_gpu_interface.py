@@ -1,0 +1,108 @@
+"""The KernelInterface path on the GPU: parameter vector -> build -> setArgs
+-> timed launch -> check, for every family and variant, in the reference's
+test-pattern mode (exact C == K, y == N, z == 2) and paranoid mode (the
+reference's host check: bit-exact for the reference-geometry kernels; FP64
+within 1e-5 for the matrix-core and wavefront-reduction kernels).  Also the
+command-line tools and the autotuner."""
+import os
+import subprocess
+
+import pytest
+
+from gatlas_amd import _capi
+from gatlas_amd.interface import BUFFER, IMAGE, MATMUL, MATVEC, MFMA, NATIVE, SAXPY, Journal, Kernel
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "gatlas_amd", "bin")
+
+CASES = [
+    # (family, dtype, vl, variant, params)
+    (MATMUL, _capi.GAT_F32, 4, BUFFER, [1, 0, 256, 256, 256, 0, 0, 16, 16, 8, 4, 5]),
+    (MATMUL, _capi.GAT_F32, 4, BUFFER, [2, 1, 128, 64, 128, 1, 1, 8, 8, 4, 4, 0]),
+    (MATMUL, _capi.GAT_F64, 2, BUFFER, [1, 1, 160, 160, 160, 0, 1, 10, 10, 4, 2, 11]),
+    (MATMUL, _capi.GAT_F32, 4, IMAGE, [1, 0, 192, 192, 192, 1, 0, 12, 12, 4, 4, 23]),
+    (MATMUL, _capi.GAT_F32, 1, MFMA, [1, 1, 300, 260, 140, 0, 0, 2, 2, 2, 2, 2]),
+    (MATMUL, _capi.GAT_F32, 1, MFMA, [3, 0, 256, 256, 128, 1, 1, 2, 2, 1, 1, 2]),
+    (MATMUL, _capi.GAT_F64, 1, MFMA, [1, 1, 128, 96, 80, 1, 0, 2, 2, 2, 2, 5]),
+    (MATVEC, _capi.GAT_F32, 4, BUFFER, [1, 1, 1024, 512, 0, 64, 8, 4, 1]),
+    (MATVEC, _capi.GAT_F64, 2, IMAGE, [1, 0, 512, 512, 1, 32, 4, 2, 3]),
+    (MATVEC, _capi.GAT_F32, 4, NATIVE, [1, 1, 1000, 1028, 0, 256, 4, 4, 1]),
+    (MATVEC, _capi.GAT_F32, 4, NATIVE, [2, 0, 2048, 512, 1, 128, 2, 4, 2]),
+    (MATVEC, _capi.GAT_F64, 2, NATIVE, [1, 1, 512, 300, 0, 64, 8, 2, 0]),
+    (SAXPY, _capi.GAT_F32, 4, BUFFER, [1, 65536, 4, 256, 0, 4, 0, 4, 1]),
+    (SAXPY, _capi.GAT_F32, 2, BUFFER, [2, 256, 64, 16, 4, 2, 4, 2, 0]),
+    (SAXPY, _capi.GAT_F64, 2, IMAGE, [1, 1 << 14, 2, 128, 0, 8, 0, 2, 3]),
+    (SAXPY, _capi.GAT_F32, 4, NATIVE, [1, 1 << 18, 4, 256, 0, 4, 0, 4, 1]),
+]
+
+
+@pytest.mark.parametrize("mode", ["pattern", "paranoid"])
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_kernel_interface_trial(ctx, ci, mode):
+    fam, dt, vl, var, params = CASES[ci]
+    print("[case]", ci, mode, params, flush=True)
+    k = Kernel(fam, dt, vl, var, ctx=ctx)
+    k.setParams(params)
+    assert k.validParams(), params
+    assert k.build(), k.print()
+    if mode == "paranoid":
+        k.paranoidCheck()
+    us = k.run(1)  # Bench::run: setArgs, launch+wait, sync, check
+    ok, err = k.checkOutput()
+    assert us > 0 and ok, (k.kernelName(), params, err)
+    if mode == "paranoid" and var in (BUFFER, IMAGE):
+        assert err == 0.0  # bit-exact against the reference's host check
+
+
+def _run(cmd, timeout=240):
+    print("[cmd]", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    return r.returncode, r.stdout, r.stderr
+
+
+def test_bench_matmul_cli_point_and_paranoid(tmp_path):
+    j = str(tmp_path / "mm.jnl")
+    rc, out, err = _run([os.path.join(BIN, "bench_matmul"), "-d", "gpu", "-j", j, "-T", "float4", "-n", "256",
+                         "-g", "16", "-y", "8", "-x", "0", "-p", "-t", "2"])
+    assert rc == 0 and "***DONE***" in out, err
+    assert "absdiff: 0" in err
+    text = open(j).read()
+    assert "matmulbufferfloat4_1_0_256_256_256_0_0_16_16_8_4_0_\t-4" in text
+
+
+def test_bench_matmul_cli_em_mfma_and_replay(tmp_path):
+    j = str(tmp_path / "em.jnl")
+    cmd = [os.path.join(BIN, "bench_matmul"), "-d", "gpu", "-j", j, "-T", "floatmfma", "-n", "1024", "-e"]
+    rc, out, err = _run(cmd)
+    assert rc == 0 and "***DONE***" in out, err
+    lines1 = open(j).read().count("\n")
+    rc, out2, err2 = _run(cmd)  # every trial is journaled: replay only
+    assert rc == 0 and "***DONE***" in out2, err2
+    assert open(j).read().count("\n") == lines1
+
+
+def test_bench_matvec_and_saxpy_cli_em(tmp_path):
+    rc, out, err = _run([os.path.join(BIN, "bench_matvec"), "-d", "gpu", "-j", str(tmp_path / "mv.jnl"), "-T",
+                         "floatnative", "-n", "4096", "-e", "-G"])
+    assert rc == 0 and "***DONE***" in out, err
+    rc, out, err = _run([os.path.join(BIN, "bench_saxpy"), "-d", "gpu", "-j", str(tmp_path / "sx.jnl"), "-T",
+                         "floatnative", "-m", str(1 << 20), "-e"])
+    assert rc == 0 and "***DONE***" in out, err
+    rc, out, err = _run([os.path.join(BIN, "bench_saxpy"), "-d", "gpu", "-j", str(tmp_path / "sx4.jnl"), "-T",
+                         "float4", "-m", str(1 << 16), "-e", "-p"])
+    assert rc == 0 and "***DONE***" in out, err
+
+
+def test_autotuner_installs_bucket_choice(ctx, tmp_path):
+    import ctypes
+    best = _capi.TileCfg()
+    gf = ctypes.c_double()
+    rc = ctx.lib.gat_tune_gemm(ctx.h, _capi.GAT_F32, 0, 0, 2048, 2048, 2048, str(tmp_path / "t.jnl").encode(), 0,
+                               ctypes.byref(best), ctypes.byref(gf))
+    assert rc == 0, _capi.last_error()
+    assert gf.value > 1000.0  # GFLOPS
+    got = _capi.TileCfg()
+    assert ctx.lib.gat_tuned_get(ctx.h, _capi.GAT_F32, 0, 0, 2048, 2048, 2048, ctypes.byref(got)) == 1
+    assert got.as_tuple() == best.as_tuple()
+    print("[tuned] 2048^3 ->", best.as_tuple(), "%.1f GFLOPS" % gf.value, flush=True)

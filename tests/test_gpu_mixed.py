@@ -1,0 +1,195 @@
+"""Mixed-precision matrix-core GEMMs on the GPU: BF16 inputs, split-BF16x3
+(FP32 inputs) and FP8 e4m3 with per-128x128-block scales.
+
+Bars (componentwise error max|C - C64| / (|alpha||op(A)||op(B)| + |beta||C0|)):
+  * BF16 and FP8 kernels against the FP64 product of the operands they were
+    given (bf16-rounded / dequantised values, exactly representable): <= 1e-5
+    -- only FP32 accumulation separates them;
+  * split-BF16x3 against the FP64 product of the original FP32 inputs: <= 1e-5;
+  * the conversions themselves: BF16 rounding against torch's bfloat16 cast,
+    FP8 encoding against torch's float8_e4m3fn cast of x / scale;
+  * the reference's all-ones test pattern gives C == K exactly in every form.
+The error of BF16 / FP8 against the original FP32 inputs is printed.
+"""
+import numpy as np
+import pytest
+import torch
+
+from gatlas_amd import _capi
+from oracle import oracle as ora
+from tests.helpers import bf16_bits_to_f32, dequant_fp8, op, uniform_host
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def test_bf16_conversion_matches_torch(ctx):
+    x = uniform_host(100003, 21, 0) * np.float32(37.5)
+    dx = ctx.to_device(x)
+    d16 = ctx.empty(x.size, np.uint16)
+    ctx.convert_bf16(dx, d16)
+    want = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(d16.download(), want)
+
+
+def test_fp8_quantizer_matches_torch_cast(ctx):
+    rows, cols = 300, 260
+    x = uniform_host(rows * cols, 22, 0) * np.float32(3.0)
+    x[5] = 0.0
+    dx = ctx.to_device(x)
+    q = ctx.empty(rows * cols, np.uint8)
+    nb = ((rows + 127) // 128) * ((cols + 127) // 128)
+    s = ctx.empty(nb, np.float32)
+    ctx.quantize_fp8(dx, rows, cols, cols, False, q, s)
+    qh, sh = q.download(), s.download()
+    # scales: smallest power of two >= amax/448 per block
+    X = x.reshape(rows, cols)
+    cb = (cols + 127) // 128
+    for b in range(nb):
+        r0, c0 = (b // cb) * 128, (b % cb) * 128
+        amax = np.abs(X[r0:r0 + 128, c0:c0 + 128]).max()
+        t = np.float32(amax) / np.float32(448.0)
+        want = 2.0 ** np.ceil(np.log2(np.float64(t)))
+        assert sh[b] == want, (b, sh[b], want)
+    srep = np.repeat(np.repeat(sh.reshape(-1, cb), 128, axis=0)[:rows], 128, axis=1)[:, :cols]
+    ref = torch.from_numpy((X / srep).astype(np.float32)).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    assert np.array_equal(qh.reshape(rows, cols), ref)
+    # transposed read gives the transposed layout
+    qt = ctx.empty(rows * cols, np.uint8)
+    st = ctx.empty(nb, np.float32)
+    ctx.quantize_fp8(dx, cols, rows, cols, True, qt, st)
+    assert np.array_equal(qt.download().reshape(cols, rows), qh.reshape(rows, cols).T)
+
+
+def _operands(M, N, K, ta, tb, seed):
+    A = uniform_host(M * K, seed, 0)
+    B = uniform_host(K * N, seed, 1)
+    C0 = uniform_host(M * N, seed, 2)
+    return A, B, C0
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(256, 256, 256), (130, 70, 200), (512, 384, 96), (1, 1, 16)])
+def test_bf16x3_accuracy(ctx, ta, tb, M, N, K):
+    _say("bf16x3", ta, tb, M, N, K)
+    A, B, C0 = _operands(M, N, K, ta, tb, 31)
+    dA, dB, dC = ctx.to_device(A), ctx.to_device(B), ctx.to_device(C0)
+    ctx.gemm(dA, dB, dC, M, N, K, transA=ta, transB=tb, alpha=1.5, beta=-0.5, dtype=_capi.GAT_BF16X3)
+    C64, MAG = ora.gemm64(A, B, M, N, K, transA=ta, transB=tb, alpha=1.5, beta=-0.5, C0=C0)
+    cw, nw, _ = ora.error_metrics(dC.download(), C64, MAG)
+    _say("bf16x3 err componentwise %.3e normwise %.3e" % (cw, nw))
+    assert cw <= TOL, (cw, nw)
+
+
+@pytest.mark.parametrize("dt", [_capi.GAT_BF16, _capi.GAT_BF16X3])
+def test_bf16_every_cfg(ctx, dt):
+    M, N, K = 264, 200, 144
+    A, B, C0 = _operands(M, N, K, 0, 0, 32)
+    for cfg in ctx.tile_cfgs(dt):
+        for ta, tb in [(0, 0), (1, 1)]:
+            _say("bf16 cfg", dt, cfg, ta, tb)
+            dA, dB, dC = ctx.to_device(A), ctx.to_device(B), ctx.empty(M * N)
+            if dt == _capi.GAT_BF16:
+                a16, b16 = ctx.empty(A.size, np.uint16), ctx.empty(B.size, np.uint16)
+                ctx.convert_bf16(dA, a16)
+                ctx.convert_bf16(dB, b16)
+                Ar, Br = bf16_bits_to_f32(a16.download()), bf16_bits_to_f32(b16.download())
+                ctx.gemm(a16, b16, dC, M, N, K, transA=ta, transB=tb, dtype=dt, cfg=cfg)
+            else:
+                Ar, Br = A, B
+                ctx.gemm(dA, dB, dC, M, N, K, transA=ta, transB=tb, dtype=dt, cfg=cfg)
+            C64, MAG = ora.gemm64(Ar, Br, M, N, K, transA=ta, transB=tb)
+            cw, _, _ = ora.error_metrics(dC.download(), C64, MAG)
+            assert cw <= TOL, (dt, cfg, ta, tb, cw)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+def test_bf16_inputs_exact_product(ctx, ta, tb):
+    M, N, K = 384, 256, 512
+    A, B, C0 = _operands(M, N, K, ta, tb, 33)
+    dA, dB, dC = ctx.to_device(A), ctx.to_device(B), ctx.empty(M * N)
+    a16, b16 = ctx.empty(A.size, np.uint16), ctx.empty(B.size, np.uint16)
+    ctx.convert_bf16(dA, a16)
+    ctx.convert_bf16(dB, b16)
+    ctx.gemm(a16, b16, dC, M, N, K, transA=ta, transB=tb, dtype=_capi.GAT_BF16)
+    got = dC.download()
+    Ar, Br = bf16_bits_to_f32(a16.download()), bf16_bits_to_f32(b16.download())
+    C64, MAG = ora.gemm64(Ar, Br, M, N, K, transA=ta, transB=tb)
+    cw, nw, _ = ora.error_metrics(got, C64, MAG)
+    assert cw <= TOL, cw
+    C64o, MAGo = ora.gemm64(A, B, M, N, K, transA=ta, transB=tb)
+    cwo, nwo, _ = ora.error_metrics(got, C64o, MAGo)
+    _say("bf16 vs original FP32 inputs: componentwise %.3e normwise %.3e" % (cwo, nwo))
+    assert cwo <= 2 ** -7
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(256, 256, 256), (200, 130, 300), (128, 640, 1024)])
+def test_fp8_exact_product_of_dequantised(ctx, ta, tb, M, N, K):
+    _say("fp8", ta, tb, M, N, K)
+    A, B, C0 = _operands(M, N, K, ta, tb, 34)
+    dA, dB, dC = ctx.to_device(A), ctx.to_device(B), ctx.to_device(C0)
+    kb = (K + 127) // 128
+    A8, B8 = ctx.empty(M * K, np.uint8), ctx.empty(N * K, np.uint8)
+    sA, sB = ctx.empty(((M + 127) // 128) * kb, np.float32), ctx.empty(((N + 127) // 128) * kb, np.float32)
+    # op(A) as M x K and op(B)^T as N x K
+    ctx.quantize_fp8(dA, M, K, M if ta else K, ta, A8, sA)
+    ctx.quantize_fp8(dB, N, K, K if tb else N, not tb, B8, sB)
+    ctx.gemm_fp8(A8, sA, B8, sB, dC, M, N, K, alpha=1.5, beta=-0.5)
+    got = dC.download()
+    Ad = dequant_fp8(A8.download(), sA.download(), M, K)          # op(A)
+    Bd = dequant_fp8(B8.download(), sB.download(), N, K)          # op(B)^T
+    C64 = 1.5 * Ad @ Bd.T - 0.5 * C0.reshape(M, N).astype(np.float64)
+    MAG = 1.5 * np.abs(Ad) @ np.abs(Bd).T + 0.5 * np.abs(C0.reshape(M, N))
+    cw, _, _ = ora.error_metrics(got, C64, MAG)
+    assert cw <= TOL, cw
+    # same through gat_gemm(dtype=FP8) from the FP32 operands
+    dC2 = ctx.to_device(C0)
+    ctx.gemm(dA, dB, dC2, M, N, K, transA=ta, transB=tb, alpha=1.5, beta=-0.5, dtype=_capi.GAT_FP8)
+    assert np.array_equal(dC2.download(), got)
+    C64o, MAGo = ora.gemm64(A, B, M, N, K, transA=ta, transB=tb, alpha=1.5, beta=-0.5, C0=C0)
+    cwo, nwo, _ = ora.error_metrics(got, C64o, MAGo)
+    _say("fp8 vs original FP32 inputs: componentwise %.3e normwise %.3e" % (cwo, nwo))
+    assert cwo <= 0.25
+
+
+@pytest.mark.parametrize("dt", [_capi.GAT_BF16X3, _capi.GAT_FP8, _capi.GAT_BF16])
+def test_mixed_test_pattern(ctx, dt):
+    M = N = K = 1024
+    ones = ctx.to_device(np.ones(M * K, np.float32))
+    dC = ctx.empty(M * N)
+    if dt == _capi.GAT_BF16:
+        o16 = ctx.empty(M * K, np.uint16)
+        ctx.convert_bf16(ones, o16)
+        ctx.gemm(o16, o16, dC, M, N, K, dtype=dt)
+    else:
+        ctx.gemm(ones, ones, dC, M, N, K, dtype=dt)
+    assert ora.check_array(dC.download(), float(K))
+
+
+@pytest.mark.parametrize("dist", ["normal", "illscaled"])
+def test_bf16x3_8192_sampled(ctx, dist):
+    # Mixed-precision config: M=N=K=8192, A, B ~ N(0,1); "illscaled"
+    # multiplies column k of A and column j of B by 10^u, u ~ U(-3, 3).
+    n = 8192
+    _say("bf16x3 8192", dist)
+    dA, dB, dC = ctx.empty(n * n), ctx.empty(n * n), ctx.empty(n * n)
+    ctx.fill_normal(dA, 0x4D495845, 0)
+    ctx.fill_normal(dB, 0x4D495845, 1)
+    if dist == "illscaled":
+        ctx.scale_columns(dA, n, n, 0x4D495845, 10)
+        ctx.scale_columns(dB, n, n, 0x4D495845, 11)
+    ctx.gemm(dA, dB, dC, n, n, n, dtype=_capi.GAT_BF16X3)
+    rows = np.arange(0, n, 409)
+    got = dC.download().reshape(n, n)[rows]
+    A, B = dA.download(), dB.download()
+    C64, MAG = ora.gemm64(A, B, n, n, n, rows=rows)
+    cw, nw, _ = ora.error_metrics(got.ravel(), C64, MAG)
+    _say("bf16x3 8192 %s: componentwise %.3e normwise %.3e" % (dist, cw, nw))
+    assert cw <= TOL
+    for a in (dA, dB, dC):
+        a.free()
