@@ -110,10 +110,10 @@ void Server::draw_trace() {
             r.out_bytes = 4 * (size_t)r.M * (size_t)r.N;
             r.foot = 4 * ((size_t)r.M * r.K + (size_t)r.K * r.N) + r.out_bytes;
         } else {
-            r.M = round_to(rng.log_uniform(cfg_.gemv_min, cfg_.gemv_max), 1, 1, cfg_.gemv_max);
-            r.N = round_to(rng.log_uniform(cfg_.gemv_min, cfg_.gemv_max), 1, 1, cfg_.gemv_max);
+            r.M = round_to(rng.log_uniform(cfg_.gemv_min, cfg_.gemv_max), 16, 16, cfg_.gemv_max);
+            r.N = round_to(rng.log_uniform(cfg_.gemv_min, cfg_.gemv_max), 16, 16, cfg_.gemv_max);
             r.K = 0;
-            r.lda = (r.N + 15) / 16 * 16;  // resident rows padded to 64 bytes
+            r.lda = r.N;
             r.ldb = 0;
             r.out_bytes = 4 * (size_t)r.M;
             r.foot = 4 * ((size_t)r.M * r.N + r.N) + r.out_bytes;

@@ -238,7 +238,7 @@ typedef struct gat_serve_cfg {
     int max_batch;                /* problems per batched launch                   */
     int max_inflight;             /* admitted, unfinished requests at once         */
     int gemm_min, gemm_max;       /* GEMM M, N, K log-uniform, multiples of 16     */
-    int gemv_min, gemv_max;       /* GEMV M, N log-uniform                         */
+    int gemv_min, gemv_max;       /* GEMV M, N log-uniform, multiples of 16        */
     int keep;                     /* keep every keep-th result (0: none) ...       */
     int keep_max;                 /* ... up to keep_max results                    */
     unsigned long long seed;
