@@ -56,4 +56,7 @@ namespace gat {
 // Thread-local last error text (also mirrored per context in the C ABI).
 void set_error(const std::string& msg);
 const char* last_error();
+// Raises `kernel`'s dynamic LDS limit to `bytes` on the current device; the
+// attribute call is made once per (kernel, device), later calls are free.
+hipError_t allow_dynamic_lds(const void* kernel, size_t bytes);
 }  // namespace gat

@@ -133,8 +133,7 @@ static int launch_compat_t(const CompatCfg& cfg, const GemmArgs& a, hipStream_t 
         return ST_INVALID;
     }
     auto f = gemm_compat_kernel<T, VL, BH>;
-    if (smem > 65536) GAT_HIP_TRY(hipFuncSetAttribute((const void*)f,
-                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    if (smem > 65536) GAT_HIP_TRY(allow_dynamic_lds((const void*)f, smem));
     dim3 grid((unsigned)(a.N / (gs * VL)), (unsigned)(a.M / (gs * BH)), (unsigned)a.batch);
     dim3 block((unsigned)gs, (unsigned)gs);
     const int gemm = !(a.alpha == 1.0 && a.beta == 0.0);

@@ -93,11 +93,7 @@ int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* wo
                         int grid, hipStream_t st) {
     const size_t smem = grouped_smem_bytes(bf16);
     const void* f = bf16 ? (const void*)grouped_gemm_kernel<1> : (const void*)grouped_gemm_kernel<0>;
-    static bool attr_set[2] = {false, false};
-    if (!attr_set[bf16 ? 1 : 0] && smem > 65536) {
-        GAT_HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        attr_set[bf16 ? 1 : 0] = true;
-    }
+    if (smem > 65536) GAT_HIP_TRY(allow_dynamic_lds(f, smem));
     if (bf16)
         hipLaunchKernelGGL(grouped_gemm_kernel<1>, dim3(grid), dim3(256), smem, st, probs, work, nwork, counter);
     else

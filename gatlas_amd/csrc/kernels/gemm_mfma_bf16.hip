@@ -63,8 +63,7 @@ static int launch_bf16_one(const GemmArgs& a, hipStream_t st) {
     else if (a.transA && !a.transB) f = gemm_bf16_kernel<WM, WN, TM, TN, BK, MODE, true, false>;
     else if (!a.transA && a.transB) f = gemm_bf16_kernel<WM, WN, TM, TN, BK, MODE, false, true>;
     else f = gemm_bf16_kernel<WM, WN, TM, TN, BK, MODE, true, true>;
-    if (smem > 65536)
-        GAT_HIP_TRY(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    if (smem > 65536) GAT_HIP_TRY(allow_dynamic_lds((const void*)f, smem));
     const int group_m = tiles_m < 8 ? tiles_m : 8;
     dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)a.batch);
     hipLaunchKernelGGL(f, grid, dim3(WM * WN * 64), smem, st, a.A, a.B, (float*)a.C, (int)a.M, (int)a.N, (int)a.K,

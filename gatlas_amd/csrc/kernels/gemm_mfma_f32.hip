@@ -65,8 +65,7 @@ static int launch_one(const GemmArgs& a, hipStream_t st) {
     else if (a.transA && !a.transB) f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, true, false>;
     else if (!a.transA && a.transB) f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, false, true>;
     else f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, true, true>;
-    if (S::SMEM > 65536)
-        GAT_HIP_TRY(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::SMEM));
+    if (S::SMEM > 65536) GAT_HIP_TRY(allow_dynamic_lds((const void*)f, S::SMEM));
     const int group_m = tiles_m < 8 ? tiles_m : 8;
     dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)a.batch);
     hipLaunchKernelGGL(f, grid, dim3(WM * WN * 64), S::SMEM, st, A, B, (float*)a.C, (int)a.M, (int)a.N,

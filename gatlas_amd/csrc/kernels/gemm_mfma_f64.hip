@@ -218,8 +218,7 @@ static int launch_one64(const GemmArgs& a, hipStream_t st) {
     else if (a.transA && !a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, true, false>;
     else if (!a.transA && a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, false, true>;
     else f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, true, true>;
-    if (smem > 65536) GAT_HIP_TRY(hipFuncSetAttribute((const void*)f,
-                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    if (smem > 65536) GAT_HIP_TRY(allow_dynamic_lds((const void*)f, smem));
     const int group_m = tiles_m < 8 ? tiles_m : 8;
     dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)a.batch);
     hipLaunchKernelGGL(f, grid, dim3(WM * WN * 64), smem, st, A, B, (double*)a.C, (int)a.M, (int)a.N,
