@@ -38,6 +38,14 @@ def parse():
     return p.parse_args()
 
 
+class _Progress(dict):
+    """The secondary figures; each finished entry is announced on stderr."""
+
+    def __setitem__(self, k, v):
+        super().__setitem__(k, v)
+        print("[bench] %s done" % k, file=sys.stderr, flush=True)
+
+
 def timed(ctx, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -152,6 +160,56 @@ def grouped_bench(ctx, dtype, reps=10):
             "loop_ms_p99": round(l99, 3), "speedup": round(l50 / g50, 3)}
 
 
+def sweep_bench(ctx, sizes=(1024, 2048, 4096, 8192)):
+    """General form C = 1.5*op(A)*op(B) - 0.5*C, all four transpose pairs,
+    square sizes, FP32 and FP64 (default tile choice): TFLOP/s per case."""
+    import numpy as np
+
+    from gatlas_amd import _capi
+    out = {}
+    for dt, name, npdt in ((_capi.GAT_F32, "f32", np.float32), (_capi.GAT_F64, "f64", np.float64)):
+        nmax = max(sizes)
+        A, B, C = ctx.empty(nmax * nmax, npdt), ctx.empty(nmax * nmax, npdt), ctx.empty(nmax * nmax, npdt)
+        for i, a in enumerate((A, B, C)):
+            ctx.fill_uniform(a, 0x6A7A5EED, i)
+        for n in sizes:
+            steps = max(2, min(20, int(4e12 / (2.0 * n ** 3))))
+            for ta in (0, 1):
+                for tb in (0, 1):
+                    ms, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n, bool(ta), bool(tb), 1.5, -0.5, dtype=dt),
+                                  steps, 1)
+                    out["%s_%d_%s%s" % (name, n, "T" if ta else "N", "T" if tb else "N")] = round(
+                        2.0 * n ** 3 / (ms * 1e-3) / 1e12, 2)
+        for a in (A, B, C):
+            a.free()
+    out["unit"] = "TFLOP/s (2n^3 per call)"
+    return out
+
+
+def tuner_bench(ctx, sizes=(1024, 2048, 4096)):
+    """The autotuner's EM search over the MFMA tile table per size bucket:
+    default-tile TFLOP/s against tuned TFLOP/s, and the tile it installed."""
+    import tempfile
+
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for n in sizes:
+            A, B, C = ctx.empty(n * n), ctx.empty(n * n), ctx.empty(n * n)
+            for i, a in enumerate((A, B, C)):
+                ctx.fill_uniform(a, 0x6A7A5EED, i)
+            ms0, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), 10, 2)
+            t0 = time.perf_counter()
+            cfg, _ = ctx.tune_gemm(n, n, n, journal=os.path.join(d, "tune_%d.journal" % n))
+            tune_s = time.perf_counter() - t0
+            ms1, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), 10, 2)
+            out["sgemm_%d" % n] = {"default_TFLOP/s": round(2.0 * n ** 3 / (ms0 * 1e-3) / 1e12, 2),
+                                   "tuned_TFLOP/s": round(2.0 * n ** 3 / (ms1 * 1e-3) / 1e12, 2),
+                                   "tile(wm,wn,tm,tn,bk,blk)": list(cfg), "search_s": round(tune_s, 2)}
+            for a in (A, B, C):
+                a.free()
+    return out
+
+
 def mixed_errors(ctx, n, nrows=32):
     """Max relative error against the FP64 oracle for every precision form on
     both input distributions (A, B ~ N(0,1); the same with every column of A
@@ -254,7 +312,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
                      "frac": round(tflops / PEAK_FP32, 4), "traffic": None},
     }
-    sec = {}
+    sec = _Progress()
     if not args.no_secondary:
         # mixed precision on the same operands (FP32 inputs resident in HBM)
         ms_x3, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n, dtype=_capi.GAT_BF16X3), 10, 2)
@@ -321,6 +379,8 @@ def main():
         sec["dgemm_8192"] = {"ms": round(ms_d, 3), "TFLOP/s": round(tf, 2), "frac_fp64": round(tf / PEAK_FP64, 4)}
         for a in (Ad, Bd, Cd):
             a.free()
+        sec["general_form_sweep"] = sweep_bench(ctx)
+        sec["tuner"] = tuner_bench(ctx)
     if not args.no_grouped:
         for dt, name in ((_capi.GAT_F32, "f32"), (_capi.GAT_BF16, "bf16")):
             sec["grouped512_" + name] = grouped_bench(ctx, dt)

@@ -60,6 +60,11 @@ class ServeStats(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class ServeArrival(ctypes.Structure):
+    _fields_ = [("t", _c_dbl), ("kind", _c_int), ("transA", _c_int), ("transB", _c_int), ("M", _c_sz),
+                ("N", _c_sz), ("K", _c_sz), ("footprint", _c_sz)]
+
+
 class ServeReq(ctypes.Structure):
     _fields_ = [("kind", _c_int), ("transA", _c_int), ("transB", _c_int), ("shape", _c_int), ("M", _c_sz),
                 ("N", _c_sz), ("K", _c_sz), ("A", _c_vp), ("lda", _c_sz), ("B", _c_vp), ("ldb", _c_sz),
@@ -70,6 +75,7 @@ class ServeReq(ctypes.Structure):
 SIGNATURES = {
     "gat_gemv_grouped": (_c_int, [_c_vp, ctypes.POINTER(GemvDesc), _c_sz]),
     "gat_serve_cfg_default": (None, [ctypes.POINTER(ServeCfg)]),
+    "gat_serve_trace": (ctypes.c_long, [ctypes.POINTER(ServeCfg), ctypes.POINTER(ServeArrival), _c_sz]),
     "gat_server_create": (_c_vp, [_c_vp, ctypes.POINTER(ServeCfg)]),
     "gat_server_run": (_c_int, [_c_vp, ctypes.POINTER(ServeStats)]),
     "gat_server_kept": (_c_sz, [_c_vp]),

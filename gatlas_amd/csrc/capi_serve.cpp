@@ -80,6 +80,28 @@ void gat_serve_cfg_default(gat_serve_cfg* cfg) {
     cfg->seed = 1;
 }
 
+long gat_serve_trace(const gat_serve_cfg* cfg, gat_serve_arrival* out, size_t cap) {
+    if (!cfg) {
+        set_error("gat_serve_trace: null configuration");
+        return GAT_ERR_INVALID;
+    }
+    if (serve_check_cfg(*cfg)) return GAT_ERR_INVALID;
+    std::vector<ServeReq> reqs;
+    size_t pool = 0;
+    serve_draw_trace(*cfg, reqs, pool);
+    for (size_t i = 0; i < reqs.size() && i < cap && out; ++i) {
+        out[i].t = reqs[i].t_arr;
+        out[i].kind = reqs[i].kind;
+        out[i].transA = reqs[i].ta;
+        out[i].transB = reqs[i].tb;
+        out[i].M = (size_t)reqs[i].M;
+        out[i].N = (size_t)reqs[i].N;
+        out[i].K = (size_t)reqs[i].K;
+        out[i].footprint = reqs[i].foot;
+    }
+    return (long)reqs.size();
+}
+
 gat_server* gat_server_create(gat_ctx* ctx, const gat_serve_cfg* cfg) {
     if (!ctx || !cfg) {
         set_error("gat_server_create: null context or configuration");

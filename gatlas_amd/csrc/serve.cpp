@@ -89,7 +89,9 @@ double pct(std::vector<double> v, double p) {
 
 }  // namespace
 
-void Server::draw_trace() {
+void Server::draw_trace() { serve_draw_trace(cfg_, reqs_, pool_elems_); }
+
+void serve_draw_trace(const gat_serve_cfg& cfg_, std::vector<ServeReq>& reqs_, size_t& pool_elems_) {
     Rng rng{cfg_.seed};
     reqs_.clear();
     double t = 0.0;
@@ -163,10 +165,10 @@ Server::~Server() {
 
 MfmaCfg Server::gemm_cfg(const ServeReq& r) const { return GroupedPlan::shape_cfg(DT_F32, r.shape); }
 
-int Server::create(Context& ctx, const gat_serve_cfg& cfg) {
+int serve_check_cfg(const gat_serve_cfg& cfg) {
     if (!(cfg.rate > 0) || !(cfg.duration > 0) || cfg.gemm_fraction < 0 || cfg.gemm_fraction > 1 ||
         cfg.streams < 1 || cfg.streams > 16 || cfg.max_batch < 1 || cfg.max_batch > 4096 || cfg.max_inflight < 1 ||
-        cfg.gemm_min < 16 || cfg.gemm_max < cfg.gemm_min || cfg.gemm_max > 16384 || cfg.gemv_min < 1 ||
+        cfg.gemm_min < 16 || cfg.gemm_max < cfg.gemm_min || cfg.gemm_max > 16384 || cfg.gemv_min < 16 ||
         cfg.gemv_max < cfg.gemv_min || cfg.gemv_max > 65536 || cfg.hbm_budget < (1u << 20) ||
         (cfg.mode != GAT_SERVE_BATCHED && cfg.mode != GAT_SERVE_LOOP)) {
         set_error("serve: invalid configuration");
@@ -176,6 +178,12 @@ int Server::create(Context& ctx, const gat_serve_cfg& cfg) {
         set_error("serve: trace longer than 5e6 requests");
         return ST_INVALID;
     }
+    return ST_OK;
+}
+
+int Server::create(Context& ctx, const gat_serve_cfg& cfg) {
+    const int vrc = serve_check_cfg(cfg);
+    if (vrc) return vrc;
     ctx_ = &ctx;
     cfg_ = cfg;
     ctx.make_current();

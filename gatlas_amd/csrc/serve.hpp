@@ -40,6 +40,13 @@ struct ServeReq {
     bool keep;
 };
 
+// The request trace of a configuration: Poisson arrivals, request shapes,
+// operand offsets into a resident pool of `pool_elems` floats, kept flags.
+// Deterministic in cfg.seed; needs no device.
+void serve_draw_trace(const gat_serve_cfg& cfg, std::vector<ServeReq>& reqs, size_t& pool_elems);
+// ST_OK, or ST_INVALID with the error text set.
+int serve_check_cfg(const gat_serve_cfg& cfg);
+
 class Server {
 public:
     ~Server();

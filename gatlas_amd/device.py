@@ -201,6 +201,15 @@ class Context:
     def grouped(self, problems, dtype=_capi.GAT_F32, use_graph=True):
         return Grouped(self, problems, dtype, use_graph)
 
+    def tune_gemm(self, M, N, K, transA=False, transB=False, dtype=_capi.GAT_F32, journal="gat_tune.journal",
+                  verbose=False):
+        """EM search over the MFMA tile table for one shape (journaled); the
+        winner is installed for the shape's size bucket.  Returns (cfg, GFLOP/s)."""
+        c, g = _capi.TileCfg(), ctypes.c_double()
+        check(self.lib.gat_tune_gemm(self.h, dtype, int(transA), int(transB), M, N, K, journal.encode(),
+                                     int(verbose), ctypes.byref(c), ctypes.byref(g)), "gat_tune_gemm")
+        return c.as_tuple(), g.value
+
     def grouped_tune(self, problems, dtype=_capi.GAT_F32, reps=2):
         descs = self.gemm_descs(problems)
         n = self.lib.gat_grouped_tune(self.h, dtype, descs, len(problems), reps)

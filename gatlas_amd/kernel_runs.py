@@ -1,0 +1,92 @@
+"""Runs named hot-path workloads a few times each, for profiling one kernel
+at its real size (e.g. under rocprofv3 --pmc), without the benchmark's other
+work:
+
+    python3 -m gatlas_amd.kernel_runs --reps 3 sgemm8192 saxpy sgemv16384
+
+Workloads (inputs U[-1,1] from the device generator):
+  sgemm8192    C = 1.5*A*B - 0.5*C, FP32, M=N=K=8192, no transpose
+  dgemm8192    the same in FP64
+  bf16x3_8192  split-BF16x3 (FP32 in/out), C = A*B
+  bf16_8192    BF16 inputs, FP32 out, C = A*B
+  fp8_8192     resident e4m3 operands with block scales, C = A*B
+  sgemv16384   y = A*x + y, FP32, 16384 x 16384 (A^T with sgemv16384t)
+  saxpy        z = 1.5*x + y, FP32, n = 2^28
+"""
+import argparse
+import sys
+
+import numpy as np
+
+from . import _capi
+from .device import Context
+
+
+def _sq(ctx, n, dt=np.float32, count=3):
+    arrs = [ctx.empty(n * n, dt) for _ in range(count)]
+    for i, a in enumerate(arrs):
+        ctx.fill_uniform(a, 0x6A7A5EED, i)
+    return arrs
+
+
+def run(ctx, name, reps):
+    n = 8192
+    if name in ("sgemm8192", "dgemm8192"):
+        A, B, C = _sq(ctx, n, np.float32 if name[0] == "s" else np.float64)
+        for _ in range(reps):
+            ctx.gemm(A, B, C, n, n, n, alpha=1.5, beta=-0.5)
+    elif name in ("bf16x3_8192", "bf16_8192"):
+        A, B, C = _sq(ctx, n)
+        if name.startswith("bf16x3"):
+            for _ in range(reps):
+                ctx.gemm(A, B, C, n, n, n, dtype=_capi.GAT_BF16X3)
+        else:
+            a16, b16 = ctx.empty(n * n, np.uint16), ctx.empty(n * n, np.uint16)
+            ctx.convert_bf16(A, a16)
+            ctx.convert_bf16(B, b16)
+            for _ in range(reps):
+                ctx.gemm(a16, b16, C, n, n, n, dtype=_capi.GAT_BF16)
+    elif name == "fp8_8192":
+        A, B, C = _sq(ctx, n)
+        nb = (n // 128) ** 2
+        A8, B8 = ctx.empty(n * n, np.uint8), ctx.empty(n * n, np.uint8)
+        sA, sB = ctx.empty(nb), ctx.empty(nb)
+        ctx.quantize_fp8(A, n, n, n, False, A8, sA)
+        ctx.quantize_fp8(B, n, n, n, True, B8, sB)
+        for _ in range(reps):
+            ctx.gemm_fp8(A8, sA, B8, sB, C, n, n, n)
+    elif name in ("sgemv16384", "sgemv16384t"):
+        m = 16384
+        A = ctx.empty(m * m)
+        x, y = ctx.empty(m), ctx.empty(m)
+        for i, a in enumerate((A, x, y)):
+            ctx.fill_uniform(a, 5, i)
+        for _ in range(reps):
+            ctx.gemv(A, x, y, m, m, transA=name.endswith("t"), alpha=1.0, beta=1.0)
+    elif name == "saxpy":
+        ns = 1 << 28
+        X, Y, Z = ctx.empty(ns), ctx.empty(ns), ctx.empty(ns)
+        ctx.fill_uniform(X, 7, 0)
+        ctx.fill_uniform(Y, 7, 1)
+        for _ in range(reps):
+            ctx.axpy(1.5, X, Y, Z)
+    else:
+        raise SystemExit("unknown workload %s" % name)
+    ctx.sync()
+    print("[kernel_runs] %s x%d done" % (name, reps), flush=True)
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser()
+    p.add_argument("--reps", type=int, default=3)
+    p.add_argument("workloads", nargs="+")
+    a = p.parse_args(argv)
+    ctx = Context(0)
+    for w in a.workloads:
+        run(ctx, w, a.reps)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -269,8 +269,18 @@ typedef struct gat_serve_req {
     size_t id;                    /* index of the request in the trace             */
 } gat_serve_req;
 
+typedef struct gat_serve_arrival {
+    double t;                     /* arrival, seconds from the start of the trace  */
+    int kind, transA, transB;     /* kind 0 GEMM, 1 GEMV                           */
+    size_t M, N, K;               /* GEMV: K = 0                                   */
+    size_t footprint;             /* admitted bytes: operands + result             */
+} gat_serve_arrival;
+
 typedef struct gat_server gat_server;
 void gat_serve_cfg_default(gat_serve_cfg *cfg);
+/* The trace a configuration replays (deterministic in cfg->seed, no device
+ * needed): writes up to cap arrivals, returns the total count or <0. */
+long gat_serve_trace(const gat_serve_cfg *cfg, gat_serve_arrival *out, size_t cap);
 /* Allocates the resident operand pool and the output arena (hbm_budget
  * bytes), draws the trace; nothing is launched yet. */
 gat_server *gat_server_create(gat_ctx *ctx, const gat_serve_cfg *cfg);

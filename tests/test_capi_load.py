@@ -58,7 +58,7 @@ def test_ctypes_structs_match_c_layout(tmp_path):
         pytest.skip("gcc not available")
     structs = {"gat_tile_cfg": _capi.TileCfg, "gat_gemm_desc": _capi.GemmDesc, "gat_gemv_desc": _capi.GemvDesc,
                "gat_serve_cfg": _capi.ServeCfg, "gat_serve_stats": _capi.ServeStats,
-               "gat_serve_req": _capi.ServeReq}
+               "gat_serve_req": _capi.ServeReq, "gat_serve_arrival": _capi.ServeArrival}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gatlas.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
