@@ -88,34 +88,41 @@ int gat_grouped_tune(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t
         a.M = (int64_t)descs[i].M;
         a.N = (int64_t)descs[i].N;
         a.K = (int64_t)descs[i].K;
-        rep.emplace(bucket_of(dtype, a), i);
+        if (a.M && a.N) rep.emplace(bucket_of(dtype, a), i);
     }
+    // A shape is judged inside a full persistent launch, as the grouped
+    // kernel runs it: the representative is replicated until its 128 x 128
+    // tiles cover kTilesPerLaunch, each replica writing its own slice of a
+    // scratch output (the caller's C is never written).
+    const size_t kTilesPerLaunch = 4 * 2 * (size_t)std::max(c.num_cu, 1);
+    const size_t kScratchBytes = (size_t)256 << 20;
     int tuned = 0;
     for (const auto& kv : rep) {
         const gat_gemm_desc& d = descs[kv.second];
-        GemmArgs a;
-        a.transA = d.transA ? 1 : 0;
-        a.transB = d.transB ? 1 : 0;
-        a.M = (int64_t)d.M;
-        a.N = (int64_t)d.N;
-        a.K = (int64_t)d.K;
-        a.A = d.A;
-        a.lda = (int64_t)d.lda;
-        a.B = d.B;
-        a.ldb = (int64_t)d.ldb;
-        a.C = d.C;
-        a.ldc = (int64_t)d.ldc;
-        a.alpha = 1.0;
-        a.beta = 0.0;  // tuning never reads C
+        const size_t tiles0 = ((d.M + 127) / 128) * ((d.N + 127) / 128);
+        const size_t cbytes = d.M * d.N * sizeof(float);
+        size_t r = (kTilesPerLaunch + tiles0 - 1) / tiles0;
+        r = std::max<size_t>(1, std::min(r, kScratchBytes / cbytes));
+        void* scratch = nullptr;
+        int rc = c.workspace(r * cbytes, &scratch);
+        if (rc) return rc;
+        std::vector<GroupedDesc> reps_d(r);
+        for (size_t i = 0; i < r; ++i)
+            reps_d[i] = GroupedDesc{d.transA ? 1 : 0, d.transB ? 1 : 0, (int64_t)d.M, (int64_t)d.N, (int64_t)d.K,
+                                    1.0, d.A, (int64_t)d.lda, d.B, (int64_t)d.ldb, 0.0,
+                                    (float*)scratch + i * d.M * d.N, (int64_t)d.N};
         float best = 0.f;
         int best_shape = 0;
         for (int s = 0; s < 2; ++s) {
-            const MfmaCfg cfg = GroupedPlan::shape_cfg(dtype, s);
-            int rc = c.gemm(dtype, a, &cfg);  // warm-up
+            const std::vector<int> shapes(r, s);
+            GroupedPlan plan;
+            rc = plan.create(c, dtype, reps_d.data(), r, false, shapes.data());
+            if (rc) return rc;
+            rc = plan.run();  // warm-up
             if (rc) return rc;
             GAT_HIP_TRY(hipEventRecord(c.ev_start, c.stream));
-            for (int r = 0; r < reps; ++r) {
-                rc = c.gemm(dtype, a, &cfg);
+            for (int k = 0; k < reps; ++k) {
+                rc = plan.run();
                 if (rc) return rc;
             }
             GAT_HIP_TRY(hipEventRecord(c.ev_stop, c.stream));

@@ -88,10 +88,11 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
     return ST_OK;
 }
 
-int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n, bool use_graph) {
+int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n, bool use_graph,
+                        const int* force_shapes) {
     std::vector<GroupedProblemDev> probs;
     std::vector<int4> work;
-    int rc = build_tables(ctx, dtype, d, n, probs, work, shapes_, flops_);
+    int rc = build_tables(ctx, dtype, d, n, probs, work, shapes_, flops_, force_shapes);
     if (rc) return rc;
     ctx_ = &ctx;
     bf16_ = dtype == DT_BF16;
@@ -101,7 +102,9 @@ int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n,
     const size_t wbytes = std::max<size_t>(1, work.size()) * sizeof(int4);
     GAT_HIP_TRY(hipMalloc(&d_probs_, pbytes));
     GAT_HIP_TRY(hipMalloc(&d_work_, wbytes));
-    GAT_HIP_TRY(hipMalloc((void**)&d_counter_, sizeof(int)));
+    GAT_HIP_TRY(hipMalloc((void**)&d_counter_, 2 * sizeof(int)));
+    // the kernel leaves its counter pair at zero; it starts there once
+    GAT_HIP_TRY(hipMemsetAsync(d_counter_, 0, 2 * sizeof(int), ctx.stream));
     if (!probs.empty())
         GAT_HIP_TRY(hipMemcpyAsync(d_probs_, probs.data(), probs.size() * sizeof(GroupedProblemDev),
                                    hipMemcpyHostToDevice, ctx.stream));
@@ -112,12 +115,13 @@ int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n,
     // two resident work-groups per CU (LDS-limited), never more than the tiles
     grid_ = std::max(1, std::min<int>(2 * (ctx.num_cu > 0 ? ctx.num_cu : 256), (int)std::max<size_t>(1, nwork_)));
     if (use_graph) {
+        rc = grouped_gemm_prepare(bf16_);  // attribute calls stay outside the capture
+        if (rc) return rc;
         GAT_HIP_TRY(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
-        int rc1 = launch_grouped_reset(d_counter_, ctx.stream);
-        int rc2 = launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_,
-                                      (int)nwork_, d_counter_, grid_, ctx.stream);
+        rc = launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, (int)nwork_,
+                                 d_counter_, grid_, ctx.stream);
         hipError_t e = hipStreamEndCapture(ctx.stream, &graph_);
-        if (rc1 || rc2) return rc1 ? rc1 : rc2;
+        if (rc) return rc;
         GAT_HIP_TRY(e);
         GAT_HIP_TRY(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
     }
@@ -130,8 +134,6 @@ int GroupedPlan::run() {
         GAT_HIP_TRY(hipGraphLaunch(exec_, ctx_->stream));
         return ST_OK;
     }
-    int rc = launch_grouped_reset(d_counter_, ctx_->stream);
-    if (rc) return rc;
     return launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, (int)nwork_,
                                d_counter_, grid_, ctx_->stream);
 }

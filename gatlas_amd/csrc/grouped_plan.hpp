@@ -27,7 +27,9 @@ class GroupedPlan {
 public:
     ~GroupedPlan();
     // dtype DT_F32 (FP32 operands) or DT_BF16 (bf16 operands), FP32 output.
-    int create(Context& ctx, int dtype, const GroupedDesc* d, size_t n, bool use_graph);
+    // force_shapes (optional): the tile shape of every problem
+    int create(Context& ctx, int dtype, const GroupedDesc* d, size_t n, bool use_graph,
+               const int* force_shapes = nullptr);
     int run();  // one persistent launch (graph replay when captured)
     size_t tiles() const { return nwork_; }
     double flops() const { return flops_; }

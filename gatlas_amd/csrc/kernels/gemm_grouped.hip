@@ -6,9 +6,12 @@
 // A grid of about two work-groups per CU loops: one work-item takes the
 // next list index from a device counter (atomicAdd), the group computes that
 // tile with the tile routine of the problem's precision, shape and transpose
-// pair, and every group leaves the loop once the list is exhausted.  The
-// counter is reset by a one-thread kernel issued before each launch (kernel
-// nodes only, so the pair can be captured in a HIP graph and replayed).
+// pair, and every group leaves the loop once the list is exhausted.
+// counter[0] is the list cursor and counter[1] counts the groups that have
+// left; the last group to leave (every other group has made its final fetch
+// by then) sets both back to zero, so consecutive launches -- and replays of
+// a captured graph holding this one kernel node -- start from a zeroed pair.
+// The pair must be zero before the first launch.
 //
 // Tile shapes: 0 = 128 x 128 (2 x 2 waves of 2 x 2 32x32 blocks),
 //              1 = 64 x 64 (2 x 2 waves of one 32x32 block).
@@ -17,8 +20,6 @@
 #include "sgemm_tile.hpp"
 
 namespace gat {
-
-__global__ void grouped_reset_kernel(int* counter) { *counter = 0; }
 
 template <bool TA, bool TB>
 __device__ __forceinline__ void f32_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0, float* smem) {
@@ -72,6 +73,13 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupedProblemD
             else bf16_tile_shape<true, true>(p, shape, t.y, t.z, hs);
         }
     }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&counter[1], 1) == (int)gridDim.x - 1) {
+            atomicExch(&counter[0], 0);
+            atomicExch(&counter[1], 0);
+        }
+    }
 }
 
 size_t grouped_smem_bytes(int bf16) {
@@ -83,17 +91,18 @@ size_t grouped_smem_bytes(int bf16) {
     return a > b ? a : b;
 }
 
-int launch_grouped_reset(int* counter, hipStream_t st) {
-    hipLaunchKernelGGL(grouped_reset_kernel, dim3(1), dim3(1), 0, st, counter);
-    GAT_HIP_TRY(hipGetLastError());
+int grouped_gemm_prepare(int bf16) {
+    const size_t smem = grouped_smem_bytes(bf16);
+    const void* f = bf16 ? (const void*)grouped_gemm_kernel<1> : (const void*)grouped_gemm_kernel<0>;
+    if (smem > 65536) GAT_HIP_TRY(allow_dynamic_lds(f, smem));
     return ST_OK;
 }
 
 int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* work, int nwork, int* counter,
                         int grid, hipStream_t st) {
     const size_t smem = grouped_smem_bytes(bf16);
-    const void* f = bf16 ? (const void*)grouped_gemm_kernel<1> : (const void*)grouped_gemm_kernel<0>;
-    if (smem > 65536) GAT_HIP_TRY(allow_dynamic_lds(f, smem));
+    const int rc = grouped_gemm_prepare(bf16);
+    if (rc) return rc;
     if (bf16)
         hipLaunchKernelGGL(grouped_gemm_kernel<1>, dim3(grid), dim3(256), smem, st, probs, work, nwork, counter);
     else

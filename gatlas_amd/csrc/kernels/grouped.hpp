@@ -23,8 +23,11 @@ struct GroupedProblemDev {
 };
 
 size_t grouped_smem_bytes(int bf16);
-int launch_grouped_reset(int* counter, hipStream_t st);
-// `work` holds nwork int4 entries {problem, m0, n0, 0}.
+// Sets the kernel's dynamic LDS limit (once); needed before graph capture.
+int grouped_gemm_prepare(int bf16);
+// `work` holds nwork int4 entries {problem, m0, n0, 0}; `counter` points to
+// two ints that are zero before the first launch and zero again after every
+// launch completes.
 int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* work, int nwork, int* counter,
                         int grid, hipStream_t st);
 

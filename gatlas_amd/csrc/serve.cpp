@@ -150,13 +150,9 @@ Server::~Server() {
         hipStreamDestroy(s);
     }
     for (hipEvent_t e : events_all_) hipEventDestroy(e);
-    for (StreamBufs& b : bufs_) {
-        if (b.h_probs) hipHostFree(b.h_probs);
-        if (b.h_work) hipHostFree(b.h_work);
-        if (b.h_gv) hipHostFree(b.h_gv);
-        if (b.d_probs) hipFree(b.d_probs);
-        if (b.d_work) hipFree(b.d_work);
-        if (b.d_gv) hipFree(b.d_gv);
+    for (Slot& b : slots_) {
+        if (b.h_stage) hipHostFree(b.h_stage);
+        if (b.d_stage) hipFree(b.d_stage);
         if (b.d_counter) hipFree(b.d_counter);
     }
     if (pool_) hipFree(pool_);
@@ -206,18 +202,21 @@ int Server::create(Context& ctx, const gat_serve_cfg& cfg) {
     const int64_t t = (cfg_.gemm_max + 63) / 64;
     work_cap_ = std::max<size_t>(1u << 18, (size_t)(t * t));
     streams_.resize(cfg_.streams);
-    bufs_.resize(cfg_.streams);
-    for (int i = 0; i < cfg_.streams; ++i) {
+    for (int i = 0; i < cfg_.streams; ++i)
         GAT_HIP_TRY(hipStreamCreateWithFlags(&streams_[i], hipStreamNonBlocking));
-        StreamBufs& b = bufs_[i];
-        if (cfg_.mode != GAT_SERVE_BATCHED) continue;
-        GAT_HIP_TRY(hipHostMalloc((void**)&b.h_probs, cfg_.max_batch * sizeof(GroupedProblemDev)));
-        GAT_HIP_TRY(hipHostMalloc((void**)&b.h_work, work_cap_ * sizeof(int4)));
-        GAT_HIP_TRY(hipHostMalloc((void**)&b.h_gv, cfg_.max_batch * sizeof(GemvProblemDev)));
-        GAT_HIP_TRY(hipMalloc(&b.d_probs, cfg_.max_batch * sizeof(GroupedProblemDev)));
-        GAT_HIP_TRY(hipMalloc(&b.d_work, work_cap_ * sizeof(int4)));
-        GAT_HIP_TRY(hipMalloc(&b.d_gv, cfg_.max_batch * sizeof(GemvProblemDev)));
-        GAT_HIP_TRY(hipMalloc((void**)&b.d_counter, sizeof(int)));
+    if (cfg_.mode == GAT_SERVE_BATCHED) {
+        // staging per batched launch in flight: GEMM descriptors followed by
+        // the tile list, or GEMV descriptors; one copy moves a batch's tables
+        stage_bytes_ = std::max(probs_bytes() + work_cap_ * sizeof(int4), cfg_.max_batch * sizeof(GemvProblemDev));
+        slots_.resize((size_t)cfg_.streams * kSlotsPerStream);
+        for (size_t i = 0; i < slots_.size(); ++i) {
+            Slot& b = slots_[i];
+            b.stream = (int)(i % (size_t)cfg_.streams);
+            GAT_HIP_TRY(hipHostMalloc((void**)&b.h_stage, stage_bytes_));
+            GAT_HIP_TRY(hipMalloc((void**)&b.d_stage, stage_bytes_));
+            GAT_HIP_TRY(hipMalloc((void**)&b.d_counter, 2 * sizeof(int)));
+            GAT_HIP_TRY(hipMemsetAsync(b.d_counter, 0, 2 * sizeof(int), ctx.stream));
+        }
     }
     GAT_HIP_TRY(hipStreamSynchronize(ctx.stream));
     return ST_OK;
@@ -270,10 +269,11 @@ int Server::launch_one(int id, int stream, Launch& L) {
     return ST_OK;
 }
 
-int Server::launch_batch(int kind, std::vector<int>& pend, int stream, Launch& L) {
-    StreamBufs& b = bufs_[stream];
-    hipStream_t st = streams_[stream];
-    L.stream = stream;
+int Server::launch_batch(int kind, std::vector<int>& pend, int slot, Launch& L) {
+    Slot& b = slots_[slot];
+    hipStream_t st = streams_[b.stream];
+    L.stream = b.stream;
+    L.slot = slot;
     L.members.clear();
     if (kind == 0) {
         std::vector<GroupedDesc> descs;
@@ -297,24 +297,25 @@ int Server::launch_batch(int kind, std::vector<int>& pend, int stream, Launch& L
         int rc = GroupedPlan::build_tables(*ctx_, DT_F32, descs.data(), descs.size(), probs, work, used, flops,
                                            shapes.data());
         if (rc) return rc;
-        std::memcpy(b.h_probs, probs.data(), probs.size() * sizeof(GroupedProblemDev));
-        std::memcpy(b.h_work, work.data(), work.size() * sizeof(int4));
-        GAT_HIP_TRY(hipMemcpyAsync(b.d_probs, b.h_probs, probs.size() * sizeof(GroupedProblemDev),
-                                   hipMemcpyHostToDevice, st));
-        GAT_HIP_TRY(hipMemcpyAsync(b.d_work, b.h_work, work.size() * sizeof(int4), hipMemcpyHostToDevice, st));
-        rc = launch_grouped_reset(b.d_counter, st);
-        if (rc) return rc;
+        // [descriptors | tile list] in one staging buffer, one copy
+        const size_t pbytes = probs.size() * sizeof(GroupedProblemDev);
+        const size_t woff = (pbytes + 15) & ~(size_t)15;
+        std::memcpy(b.h_stage, probs.data(), pbytes);
+        std::memcpy(b.h_stage + woff, work.data(), work.size() * sizeof(int4));
+        GAT_HIP_TRY(hipMemcpyAsync(b.d_stage, b.h_stage, woff + work.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                   st));
         const int grid = (int)std::max<size_t>(1, std::min<size_t>(2 * (size_t)std::max(ctx_->num_cu, 1), work.size()));
-        rc = launch_grouped_gemm(0, (const GroupedProblemDev*)b.d_probs, (const int4*)b.d_work, (int)work.size(),
-                                 b.d_counter, grid, st);
+        rc = launch_grouped_gemm(0, (const GroupedProblemDev*)b.d_stage, (const int4*)(b.d_stage + woff),
+                                 (int)work.size(), b.d_counter, grid, st);
         if (rc) return rc;
         pend.erase(pend.begin(), pend.begin() + take);
     } else {
         const size_t take = std::min<size_t>(pend.size(), (size_t)cfg_.max_batch);
+        GemvProblemDev* gv = (GemvProblemDev*)b.h_stage;
         int64_t blocks = 0;
         for (size_t i = 0; i < take; ++i) {
             const ServeReq& r = reqs_[pend[i]];
-            GemvProblemDev& g = b.h_gv[i];
+            GemvProblemDev& g = gv[i];
             g.A = pool_ptr(r.aoff);
             g.x = pool_ptr(r.boff);
             g.y = out_ptr(r);
@@ -328,8 +329,8 @@ int Server::launch_batch(int kind, std::vector<int>& pend, int stream, Launch& L
             blocks += gemv_grouped_blocks(r.M);
             L.members.push_back(pend[i]);
         }
-        GAT_HIP_TRY(hipMemcpyAsync(b.d_gv, b.h_gv, take * sizeof(GemvProblemDev), hipMemcpyHostToDevice, st));
-        int rc = launch_grouped_gemv((const GemvProblemDev*)b.d_gv, (int)take, blocks, st);
+        GAT_HIP_TRY(hipMemcpyAsync(b.d_stage, b.h_stage, take * sizeof(GemvProblemDev), hipMemcpyHostToDevice, st));
+        int rc = launch_grouped_gemv((const GemvProblemDev*)b.d_stage, (int)take, blocks, st);
         if (rc) return rc;
         pend.erase(pend.begin(), pend.begin() + take);
     }
@@ -377,8 +378,8 @@ int Server::run(gat_serve_stats* out) {
                     ++done;
                 }
                 events_free_.push_back(L.ev);
+                if (batched) slots_[L.slot].busy = false;
                 inflight[s].pop_front();
-                if (batched) bufs_[s].busy = false;
                 progress = true;
             }
         }
@@ -406,20 +407,23 @@ int Server::run(gat_serve_stats* out) {
         if (batched) {
             for (int kind = 1; kind >= 0; --kind) {  // GEMV first: short, latency-bound
                 if (pend[kind].empty()) continue;
+                // a free slot on an idle stream first (the batch overlaps the
+                // running ones), else any free slot (it queues on its stream)
                 int s = -1;
-                for (size_t i = 0; i < streams_.size(); ++i)
-                    if (!bufs_[i].busy) {
-                        s = (int)i;
-                        break;
-                    }
+                for (int pass = 0; pass < 2 && s < 0; ++pass)
+                    for (size_t i = 0; i < slots_.size(); ++i)
+                        if (!slots_[i].busy && (pass == 1 || inflight[slots_[i].stream].empty())) {
+                            s = (int)i;
+                            break;
+                        }
                 if (s < 0) break;
                 Launch L;
                 int rc = launch_batch(kind, pend[kind], s, L);
                 if (rc) return rc;
-                bufs_[s].busy = true;
+                slots_[s].busy = true;
                 launched_reqs += L.members.size();
                 ++launches;
-                inflight[s].push_back(std::move(L));
+                inflight[L.stream].push_back(std::move(L));
                 progress = true;
             }
         } else {

@@ -59,24 +59,28 @@ public:
 private:
     struct Launch {
         int stream;
+        int slot = -1;  // staging slot of a batched launch
         hipEvent_t ev;
         std::vector<int> members;
     };
-    struct StreamBufs {  // per-stream staging for one batched launch in flight
-        GroupedProblemDev* h_probs = nullptr;
-        int4* h_work = nullptr;
-        GemvProblemDev* h_gv = nullptr;
-        void* d_probs = nullptr;
-        void* d_work = nullptr;
-        void* d_gv = nullptr;
+    // Staging of one batched launch in flight: pinned host tables, their
+    // device copy and the grouped kernel's counter pair.
+    struct Slot {
+        int stream = 0;
+        char* h_stage = nullptr;
+        char* d_stage = nullptr;
         int* d_counter = nullptr;
         bool busy = false;
     };
+    static constexpr int kSlotsPerStream = 2;
+    size_t probs_bytes() const {
+        return ((size_t)cfg_.max_batch * sizeof(GroupedProblemDev) + 15) & ~(size_t)15;
+    }
     void draw_trace();
     float* pool_ptr(size_t off) const { return pool_ + off; }
     float* out_ptr(const ServeReq& r) const { return (float*)((char*)arena_base_ + r.out_off); }
     MfmaCfg gemm_cfg(const ServeReq& r) const;
-    int launch_batch(int kind, std::vector<int>& pend, int stream, Launch& L);
+    int launch_batch(int kind, std::vector<int>& pend, int slot, Launch& L);
     int launch_one(int id, int stream, Launch& L);
     hipEvent_t get_event();
 
@@ -89,7 +93,8 @@ private:
     void* arena_base_ = nullptr;
     DeviceArena arena_;
     std::vector<hipStream_t> streams_;
-    std::vector<StreamBufs> bufs_;
+    std::vector<Slot> slots_;
+    size_t stage_bytes_ = 0;
     std::vector<hipEvent_t> events_free_, events_all_;
     size_t work_cap_ = 0;
 };

@@ -202,8 +202,10 @@ int gat_grouped_shape(gat_grouped *g, size_t i);
 void gat_grouped_destroy(gat_grouped *g);
 /* create + run + synchronise + destroy. */
 int gat_gemm_grouped(gat_ctx *ctx, int dtype, const gat_gemm_desc *descs, size_t count);
-/* Times both tile shapes on one problem per size bucket and installs the faster
- * in the context's tuned table; returns the number of buckets tuned. */
+/* Times both tile shapes for one problem per size bucket, inside a full
+ * persistent launch of replicas of that problem (reading its A and B, writing
+ * library scratch: the caller's C is not touched), and installs the faster in
+ * the context's tuned table; returns the number of buckets tuned. */
 int gat_grouped_tune(gat_ctx *ctx, int dtype, const gat_gemm_desc *descs, size_t count, int reps);
 
 /* Grouped GEMV (FP32, A row-major M x N): y = alpha*A*x + beta*y for many

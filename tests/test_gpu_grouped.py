@@ -130,6 +130,19 @@ def test_grouped_graph_replay(ctx):
         ctx.sync()
         for w, p in zip(want, probs):
             assert np.array_equal(p["C"].download().view(np.uint32), w.view(np.uint32)), rep
+    # back-to-back launches with no host wait, outputs clobbered before each:
+    # a launch that did not start from a zeroed counter would leave the
+    # clobbered values, and so would every launch after it
+    for i in range(3):
+        for p in probs:
+            ctx.fill_uniform(p["C"], 97, 2 * i)
+        graph.run()
+        for p in probs:
+            ctx.fill_uniform(p["C"], 97, 2 * i + 1)
+        eager.run()
+    ctx.sync()
+    for w, p in zip(want, probs):
+        assert np.array_equal(p["C"].download().view(np.uint32), w.view(np.uint32))
     _check_oracle(dims, host, probs)
 
 

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from gatlas_amd import _capi
+from gatlas_amd.device import Context
 from oracle import oracle as ora
 from tests.helpers import op, uniform_host
 
@@ -233,12 +234,8 @@ def test_sgemm_mfma_default(ctx, ta, tb, M, N, K):
     assert cw <= F32_TOL, (cw, nw)
 
 
-@pytest.mark.parametrize("ci", range(16))
-def test_sgemm_mfma_every_cfg(ctx, ci):
-    cfgs = ctx.tile_cfgs(_capi.GAT_F32)
-    if ci >= len(cfgs):
-        pytest.skip("fewer tile configurations")
-    cfg = cfgs[ci]
+@pytest.mark.parametrize("cfg", Context.tile_cfgs(_capi.GAT_F32), ids=str)
+def test_sgemm_mfma_every_cfg(ctx, cfg):
     for ta, tb in [(0, 0), (1, 0), (0, 1), (1, 1)]:
         for (M, N, K) in [(264, 136, 72), (512, 512, 256)]:
             _say("sgemm cfg", cfg, ta, tb, M, N, K)
