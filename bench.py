@@ -123,6 +123,20 @@ def grouped_bench(ctx, dtype, reps=10):
         probs.append(dict(transA=ta, transB=tb, M=M, N=N, K=K, alpha=1.0, beta=0.0, A=dA, B=dB, C=dC))
     flops = sum(2.0 * p["M"] * p["N"] * p["K"] for p in probs)
     ctx.sync()
+
+    # the per-problem loop first, with the single-launch default tile rule
+    def per_problem():
+        for p in probs:
+            ctx.gemm(p["A"], p["B"], p["C"], p["M"], p["N"], p["K"], bool(p["transA"]), bool(p["transB"]),
+                     1.0, 0.0, dtype=dtype)
+    ctx.tuned_clear()
+    per_problem()
+    ctx.sync()
+    loop = []
+    for _ in range(reps):
+        ctx.timer_start()
+        per_problem()
+        loop.append(ctx.timer_stop())
     t0 = time.perf_counter()
     nb = ctx.grouped_tune(probs, dtype)
     tune_s = time.perf_counter() - t0
@@ -134,18 +148,7 @@ def grouped_bench(ctx, dtype, reps=10):
         ctx.timer_start()
         g.run()
         gl.append(ctx.timer_stop())
-    loop = []
-
-    def per_problem():
-        for p in probs:
-            ctx.gemm(p["A"], p["B"], p["C"], p["M"], p["N"], p["K"], bool(p["transA"]), bool(p["transB"]),
-                     1.0, 0.0, dtype=dtype)
-    per_problem()
-    ctx.sync()
-    for _ in range(reps):
-        ctx.timer_start()
-        per_problem()
-        loop.append(ctx.timer_stop())
+    ctx.tuned_clear()
     info = g.info()
     g.close()
     for a in arrays:

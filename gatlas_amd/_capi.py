@@ -169,6 +169,7 @@ SIGNATURES = {
                                ctypes.POINTER(TileCfg), ctypes.POINTER(_c_dbl)]),
     "gat_tuned_set": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, ctypes.POINTER(TileCfg)]),
     "gat_tuned_get": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, ctypes.POINTER(TileCfg)]),
+    "gat_tuned_clear": (_c_int, [_c_vp]),
 }
 
 _lib = None

@@ -289,7 +289,8 @@ int gat_search_em(gat_kernel* k, const char* journal_path, size_t max_workgroup,
         app = local.get();
     }
     Journal journal(journal_path);
-    Bench bench(*app, *k->k, journal, true);
+    Bench bench(*app, *k->k, journal, false);
+    QuietReports quiet(true);  // a library call: the caller's stdout stays clean
     LoopOpts o;
     o.numberTrials = ntrials ? ntrials : 1;
     o.paranoid = paranoid != 0;
@@ -321,6 +322,7 @@ int gat_tune_gemm(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, siz
     if (!k) return bad("gat_tune_gemm: precision form has no matrix-core kernel");
     KernelBaseMatmul* m = as_matmul(k.get());
     DeviceApp app(&ctx->impl);
+    QuietReports quiet(verbose == 0);
     Journal journal(journal_path);
     Bench bench(app, *k, journal, verbose != 0);
     LoopOpts o;
@@ -389,6 +391,12 @@ int gat_tuned_get(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, siz
     if (!tuned) c = Context::default_cfg(dtype, a);
     *out = gat_tile_cfg{c.waves_m, c.waves_n, c.tm, c.tn, c.bk, c.blk};
     return tuned ? 1 : 0;
+}
+
+int gat_tuned_clear(gat_ctx* ctx) {
+    if (!ctx) return bad("null context");
+    ctx->impl.clear_tuned();
+    return GAT_OK;
 }
 
 }  // extern "C"

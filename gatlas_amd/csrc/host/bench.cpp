@@ -11,6 +11,18 @@
 
 namespace gat {
 
+static thread_local bool g_quiet = false;
+
+std::ostream& report() {
+    static std::ostream null_stream(nullptr);  // no buffer: every write is dropped
+    return g_quiet ? null_stream : std::cout;
+}
+
+QuietReports::QuietReports(bool on) : prev_(g_quiet) {
+    if (on) g_quiet = true;
+}
+QuietReports::~QuietReports() { g_quiet = prev_; }
+
 // ------------------------------------------------------------- Journal
 std::string Journal::key(const KernelInterface& k, const std::vector<size_t>& params) {
     std::ostringstream s;
@@ -135,10 +147,10 @@ void benchInit(const std::vector<std::vector<size_t>>& pargs, std::vector<bool>&
 }
 
 static void print_args(const std::vector<size_t>& args, const std::vector<size_t>& extra) {
-    for (size_t i = 0; i < args.size(); ++i) std::cout << args[i] << (i + 1 < args.size() ? " " : "");
-    std::cout << "\t(";
-    for (size_t i = 0; i < extra.size(); ++i) std::cout << extra[i] << (i + 1 < extra.size() ? " " : "");
-    std::cout << ")" << std::endl;
+    for (size_t i = 0; i < args.size(); ++i) report() << args[i] << (i + 1 < args.size() ? " " : "");
+    report() << "\t(";
+    for (size_t i = 0; i < extra.size(); ++i) report() << extra[i] << (i + 1 < extra.size() ? " " : "");
+    report() << ")" << std::endl;
 }
 
 size_t benchLoop(size_t trialNumber, KernelInterface& kernel, Bench& bench, Journal* journal,
@@ -154,12 +166,12 @@ size_t benchLoop(size_t trialNumber, KernelInterface& kernel, Bench& bench, Jour
         if (!ok[j]) continue;
         const int memoState = journal ? journal->memoRunState(kernel, args) : Journal::MISSING;
         if (needDummy && memoState == Journal::MISSING) {
-            std::cout << "[dummy run] ";
+            report() << "[dummy run] ";
             bench.run(1, args, toDevice, fromDevice, printDebug);
-            std::cout << std::endl;
+            report() << std::endl;
             needDummy = false;
         }
-        if (printStatus) std::cout << "[trial " << trialNumber << "] ";
+        if (printStatus) report() << "[trial " << trialNumber << "] ";
         size_t us;
         if (memoState == Journal::MISSING) {
             us = bench.run(1, args, toDevice, fromDevice, printDebug);
@@ -169,14 +181,14 @@ size_t benchLoop(size_t trialNumber, KernelInterface& kernel, Bench& bench, Jour
             kernel.setParams(args);
         } else {
             // crashed while building or setting arguments, or hung while running
-            if (printStatus) std::cout << "bad";
+            if (printStatus) report() << "bad";
             us = 0;
             kernel.setParams(args);
         }
         extra[j] = kernel.extraParamDetail();
         if (us == 0) {
             if (printStatus || !journal) {
-                std::cout << "\t";
+                report() << "\t";
                 print_args(args, extra[j]);
             }
             ok[j] = false;
@@ -195,7 +207,7 @@ size_t benchLoop(size_t trialNumber, KernelInterface& kernel, Bench& bench, Jour
             var[j] += ((double)trialNumber / (trialNumber + 1)) * delta * delta;
         }
         if (printStatus) {
-            std::cout << us << "\t";
+            report() << us << "\t";
             print_args(args, extra[j]);
         }
     }
@@ -249,7 +261,7 @@ void printBench(size_t numberTrials, const std::vector<std::vector<size_t>>& par
     size_t count = 0;
     for (const auto& kv : byTime) {
         const size_t idx = kv.second;
-        std::cout << "[" << count++ << "] " << kv.first << " usec"
+        report() << "[" << count++ << "] " << kv.first << " usec"
                   << "\tavg: " << avg[idx] << "\tstddev: " << std::sqrt(var[idx] / numberTrials) << "\t";
         print_args(pargs[idx], extra[idx]);
     }

@@ -67,6 +67,19 @@ private:
     bool print_;
 };
 
+// Report stream of the benchmark loops: standard output, as the command-line
+// tools print it, unless the calling thread has silenced it.
+std::ostream& report();
+// Silences report() on the calling thread while it lives (when `on`).
+class QuietReports {
+public:
+    explicit QuietReports(bool on);
+    ~QuietReports();
+
+private:
+    bool prev_;
+};
+
 namespace AppUtil {
 
 void benchInit(const std::vector<std::vector<size_t>>& pargs, std::vector<bool>& ok, std::vector<size_t>& time,

@@ -44,8 +44,11 @@ struct StageIO<MODE_BF16, ROWS, BK, NT, TRANS> {
     static constexpr int LDK = BK + 8;
     u32x4 reg[NV];
 
+    // full: the whole tile is in bounds; otherwise each vector is checked on
+    // its own (out-of-range elements read as zero, an in-range vector is one
+    // 16-byte load when `vec`).
     __device__ __forceinline__ void load(const __bf16* __restrict__ src, int64_t ld, int r0, int k0, int rlim,
-                                         int klim, bool full, int tid) {
+                                         int klim, bool full, bool vec, int tid) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             const int idx = q * NT + tid;
@@ -58,7 +61,8 @@ struct StageIO<MODE_BF16, ROWS, BK, NT, TRANS> {
                 gr = r0 + (idx / BK) * W;
             }
             const __bf16* p = !TRANS ? src + (int64_t)gr * ld + gk : src + (int64_t)gk * ld + gr;
-            if (full) {
+            const bool vin = !TRANS ? (gr < rlim && gk + W <= klim) : (gk < klim && gr + W <= rlim);
+            if (full || (vec && vin)) {
                 reg[q] = *(const u32x4*)p;
             } else {
                 __bf16 t[W];
@@ -99,7 +103,7 @@ struct StageIO<MODE_X3, ROWS, BK, NT, TRANS> {
     f4v reg[NV];
 
     __device__ __forceinline__ void load(const float* __restrict__ src, int64_t ld, int r0, int k0, int rlim,
-                                         int klim, bool full, int tid) {
+                                         int klim, bool full, bool vec, int tid) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             const int idx = q * NT + tid;
@@ -112,7 +116,8 @@ struct StageIO<MODE_X3, ROWS, BK, NT, TRANS> {
                 gr = r0 + (idx / BK) * W;
             }
             const float* p = !TRANS ? src + (int64_t)gr * ld + gk : src + (int64_t)gk * ld + gr;
-            if (full) {
+            const bool vin = !TRANS ? (gr < rlim && gk + W <= klim) : (gk < klim && gr + W <= rlim);
+            if (full || (vec && vin)) {
                 reg[q] = *(const f4v*)p;
             } else {
 #pragma unroll
@@ -189,8 +194,8 @@ __device__ __forceinline__ void bf16_tile(const void* __restrict__ Av, const voi
     auto gload = [&](int t) {
         const int k0 = t * BK;
         const bool full = full_mn && (k0 + BK <= K);
-        ioa.load(A, lda, m0, k0, M, K, full, tid);
-        iob.load(B, ldb, n0, k0, N, K, full, tid);
+        ioa.load(A, lda, m0, k0, M, K, full, vec_ok, tid);
+        iob.load(B, ldb, n0, k0, N, K, full, vec_ok, tid);
     };
     auto sstore = [&](__bf16* buf) {
         ioa.store(buf, buf + PLANE_A + PLANE_B, tid);

@@ -65,15 +65,19 @@ struct TileIO {
     static_assert(NV >= 1 && NV * 4 * NT == ROWS * BK, "tile must split evenly into float4 per thread");
     static constexpr int LDK = BK + 4;
 
+    // full: the whole tile is in bounds (no checks).  Otherwise each vector
+    // is checked on its own: out-of-range elements read as zero, a vector
+    // wholly in range is still one 16-byte load when `vec` (aligned source).
     __device__ static __forceinline__ void load(f4v (&reg)[NV], const float* __restrict__ src, int64_t ld,
-                                                int r0, int k0, int rows_lim, int k_lim, bool full, int tid) {
+                                                int r0, int k0, int rows_lim, int k_lim, bool full, bool vec,
+                                                int tid) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             const int idx = q * NT + tid;
             if (!TRANS) {
                 const int r = idx / (BK / 4), kq = (idx % (BK / 4)) * 4;
                 const int gr = r0 + r, gk = k0 + kq;
-                if (full) {
+                if (full || (vec && gr < rows_lim && gk + 4 <= k_lim)) {
                     reg[q] = *(const f4v*)(src + (int64_t)gr * ld + gk);
                 } else {
 #pragma unroll
@@ -83,7 +87,7 @@ struct TileIO {
             } else {
                 const int k = idx % BK, rq = (idx / BK) * 4;
                 const int gk = k0 + k, gr = r0 + rq;
-                if (full) {
+                if (full || (vec && gk < k_lim && gr + 4 <= rows_lim)) {
                     reg[q] = *(const f4v*)(src + (int64_t)gk * ld + gr);
                 } else {
 #pragma unroll
@@ -150,8 +154,8 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
     auto gload = [&](int t) {
         const int k0 = t * BK;
         const bool full = full_mn && (k0 + BK <= K);
-        IoA::load(ra, A, lda, m0, k0, M, K, full, tid);
-        IoB::load(rb, B, ldb, n0, k0, N, K, full, tid);
+        IoA::load(ra, A, lda, m0, k0, M, K, full, vec_ok, tid);
+        IoB::load(rb, B, ldb, n0, k0, N, K, full, vec_ok, tid);
     };
 
     gload(0);

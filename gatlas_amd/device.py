@@ -210,6 +210,9 @@ class Context:
                                      int(verbose), ctypes.byref(c), ctypes.byref(g)), "gat_tune_gemm")
         return c.as_tuple(), g.value
 
+    def tuned_clear(self):
+        check(self.lib.gat_tuned_clear(self.h), "gat_tuned_clear")
+
     def grouped_tune(self, problems, dtype=_capi.GAT_F32, reps=2):
         descs = self.gemm_descs(problems)
         n = self.lib.gat_grouped_tune(self.h, dtype, descs, len(problems), reps)

@@ -397,6 +397,8 @@ int gat_tuned_set(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, siz
                   const gat_tile_cfg *cfg);
 int gat_tuned_get(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, size_t N, size_t K,
                   gat_tile_cfg *out);
+/* Forget every tuned choice: all shapes use the default tile rule again. */
+int gat_tuned_clear(gat_ctx *ctx);
 
 #ifdef __cplusplus
 }

@@ -145,3 +145,16 @@ def test_matvec_and_saxpy_em_replay(tmp_path):
                       for p in cs])
     best = ks.em_search(path, 1 << 18, 4, vl=4)
     assert best == want
+
+
+def test_library_search_keeps_stdout_clean(tmp_path, capfd):
+    # the command-line tools print every trial; the library entry point
+    # prints nothing, so a caller's own output (one JSON line) stays intact
+    k, c = _cands()
+    path = str(tmp_path / "j.txt")
+    _write(path, k, [(p, [-1, -2, -3, -4, _time(p)]) for p in c])
+    capfd.readouterr()
+    best = k.em_search(path, M, N, K, vl=4)
+    out = capfd.readouterr().out
+    assert best is not None
+    assert out == ""
