@@ -38,6 +38,18 @@ def parse():
     return p.parse_args()
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_pmc.md), scaled by the
+    same counters' reading of a SAXPY of known size; None when absent."""
+    import json
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            return int(json.load(f)[key]["hbm_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 class _Progress(dict):
     """The secondary figures; each finished entry is announced on stderr."""
 
@@ -313,7 +325,8 @@ def main():
                    "pct_peak_fp32": round(100.0 * tflops / PEAK_FP32, 2),
                    "wall_s_timed_region": round(wall, 4)},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
-                     "frac": round(tflops / PEAK_FP32, 4), "traffic": None},
+                     "frac": round(tflops / PEAK_FP32, 4),
+                     "traffic": measured_traffic("sgemm_8192_general_nn") if n == 8192 else None},
     }
     sec = _Progress()
     if not args.no_secondary:
