@@ -139,7 +139,7 @@ int Context::num_cfgs(int dtype) {
         case DT_F32: return mfma_f32_num_cfgs();
         case DT_F64: return mfma_f64_num_cfgs();
         case DT_BF16:
-        case DT_BF16X3: return mfma_bf16_num_cfgs();
+        case DT_BF16X3: return mfma_bf16_num_cfgs(dtype);
         case DT_FP8: return 1;
     }
     return 0;
@@ -150,7 +150,7 @@ MfmaCfg Context::cfg_at(int dtype, int i) {
         case DT_F32: return mfma_f32_cfg(i);
         case DT_F64: return mfma_f64_cfg(i);
         case DT_BF16:
-        case DT_BF16X3: return mfma_bf16_cfg(i);
+        case DT_BF16X3: return mfma_bf16_cfg(dtype, i);
         case DT_FP8: return kFp8Cfg;
     }
     return MfmaCfg{0, 0, 0, 0, 0, 0};
@@ -161,7 +161,7 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
         case DT_F32: return mfma_f32_has(c);
         case DT_F64: return mfma_f64_has(c);
         case DT_BF16:
-        case DT_BF16X3: return mfma_bf16_has(c);
+        case DT_BF16X3: return mfma_bf16_has(dtype, c);
         case DT_FP8:
             return c.waves_m == kFp8Cfg.waves_m && c.waves_n == kFp8Cfg.waves_n && c.tm == kFp8Cfg.tm &&
                    c.tn == kFp8Cfg.tn && c.bk == kFp8Cfg.bk && c.blk == kFp8Cfg.blk;

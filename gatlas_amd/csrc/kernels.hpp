@@ -102,9 +102,10 @@ bool compat_has(int dtype, const CompatCfg& cfg);
 // BF16 forms (gemm_mfma_bf16.hip): dtype DT_BF16 (bf16 inputs) or
 // DT_BF16X3 (FP32 inputs split into bf16 hi/lo, three products).
 int launch_gemm_mfma_bf16(int dtype, const MfmaCfg& cfg, const GemmArgs& a, hipStream_t st);
-int mfma_bf16_num_cfgs();
-MfmaCfg mfma_bf16_cfg(int i);
-bool mfma_bf16_has(const MfmaCfg& cfg);
+// Tile tables per form (dtype DT_BF16 or DT_BF16X3).
+int mfma_bf16_num_cfgs(int dtype);
+MfmaCfg mfma_bf16_cfg(int dtype, int i);
+bool mfma_bf16_has(int dtype, const MfmaCfg& cfg);
 int launch_convert_f32_bf16(const float* x, void* y, size_t n, hipStream_t st);
 
 // FP8 e4m3 (gemm_fp8.hip): block quantisation of a rows x cols operand whose

@@ -62,7 +62,9 @@ struct StageIO<MODE_BF16, ROWS, BK, NT, TRANS> {
             }
             const __bf16* p = !TRANS ? src + (int64_t)gr * ld + gk : src + (int64_t)gk * ld + gr;
             const bool vin = !TRANS ? (gr < rlim && gk + W <= klim) : (gk < klim && gr + W <= rlim);
-            if (full || (vec && vin)) {
+            if (full) {  // uniform: interior tiles
+                reg[q] = *(const u32x4*)p;
+            } else if (vec && vin) {
                 reg[q] = *(const u32x4*)p;
             } else {
                 __bf16 t[W];
@@ -117,7 +119,9 @@ struct StageIO<MODE_X3, ROWS, BK, NT, TRANS> {
             }
             const float* p = !TRANS ? src + (int64_t)gr * ld + gk : src + (int64_t)gk * ld + gr;
             const bool vin = !TRANS ? (gr < rlim && gk + W <= klim) : (gk < klim && gr + W <= rlim);
-            if (full || (vec && vin)) {
+            if (full) {  // uniform: interior tiles
+                reg[q] = *(const f4v*)p;
+            } else if (vec && vin) {
                 reg[q] = *(const f4v*)p;
             } else {
 #pragma unroll

@@ -22,28 +22,50 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_bf16_kernel(
                                                tni * S::BN, vec_ok != 0, smemh);
 }
 
-// Instantiated configurations (WM, WN, TM, TN, BK); blocks are 32x32.
-#define GAT_BF16_CFGS(X) \
-    X(2, 2, 2, 2, 32)    \
-    X(2, 2, 2, 2, 64)    \
-    X(2, 2, 4, 2, 32)    \
-    X(2, 2, 2, 4, 32)    \
-    X(2, 2, 1, 1, 64)    \
-    X(1, 1, 2, 2, 64)
+// Instantiated configurations (WM, WN, TM, TN, BK, forms); blocks are
+// 32x32; forms: 1 = BF16 inputs, 2 = split-BF16x3, 3 = both.  The 8-wave
+// 256 x 256 tiles keep 0.75 KiB of LDS reads per MFMA (4 x 2 blocks per
+// wave); in the x3 form they stage two planes, so only BK = 16 fits.
+#define GAT_BF16_CFGS(X)  \
+    X(2, 2, 2, 2, 32, 3)  \
+    X(2, 2, 2, 2, 64, 3)  \
+    X(2, 2, 4, 2, 32, 3)  \
+    X(2, 2, 2, 4, 32, 3)  \
+    X(2, 2, 1, 1, 64, 3)  \
+    X(1, 1, 2, 2, 64, 3)  \
+    X(2, 4, 4, 2, 32, 1)  \
+    X(2, 4, 4, 2, 64, 1)  \
+    X(2, 4, 4, 2, 16, 2)
 
-static const MfmaCfg kBf16Cfgs[] = {
-#define X(wm, wn, tm, tn, bk) {wm, wn, tm, tn, bk, 32},
+struct Bf16CfgEntry {
+    MfmaCfg cfg;
+    int forms;
+};
+static const Bf16CfgEntry kBf16Cfgs[] = {
+#define X(wm, wn, tm, tn, bk, forms) {{wm, wn, tm, tn, bk, 32}, forms},
     GAT_BF16_CFGS(X)
 #undef X
 };
 
-int mfma_bf16_num_cfgs() { return (int)(sizeof(kBf16Cfgs) / sizeof(kBf16Cfgs[0])); }
-MfmaCfg mfma_bf16_cfg(int i) { return kBf16Cfgs[i]; }
-bool mfma_bf16_has(const MfmaCfg& c) {
-    for (const MfmaCfg& k : kBf16Cfgs)
-        if (k.waves_m == c.waves_m && k.waves_n == c.waves_n && k.tm == c.tm && k.tn == c.tn && k.bk == c.bk &&
-            k.blk == c.blk)
+static int form_bit(int dtype) { return dtype == DT_BF16X3 ? 2 : 1; }
+
+int mfma_bf16_num_cfgs(int dtype) {
+    int n = 0;
+    for (const Bf16CfgEntry& e : kBf16Cfgs) n += (e.forms & form_bit(dtype)) != 0;
+    return n;
+}
+MfmaCfg mfma_bf16_cfg(int dtype, int i) {
+    for (const Bf16CfgEntry& e : kBf16Cfgs)
+        if ((e.forms & form_bit(dtype)) && i-- == 0) return e.cfg;
+    return MfmaCfg{0, 0, 0, 0, 0, 0};
+}
+bool mfma_bf16_has(int dtype, const MfmaCfg& c) {
+    for (const Bf16CfgEntry& e : kBf16Cfgs) {
+        const MfmaCfg& k = e.cfg;
+        if ((e.forms & form_bit(dtype)) && k.waves_m == c.waves_m && k.waves_n == c.waves_n && k.tm == c.tm &&
+            k.tn == c.tn && k.bk == c.bk && k.blk == c.blk)
             return true;
+    }
     return false;
 }
 
@@ -77,11 +99,17 @@ int launch_gemm_mfma_bf16(int dtype, const MfmaCfg& c, const GemmArgs& a, hipStr
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return ST_OK;
     if (a.M > INT32_MAX || a.N > INT32_MAX || a.K > INT32_MAX) return ST_INVALID;
     if (dtype != DT_BF16 && dtype != DT_BF16X3) return ST_INVALID;
-#define X(p_wm, p_wn, p_tm, p_tn, p_bk)                                                            \
-    if (c.waves_m == p_wm && c.waves_n == p_wn && c.tm == p_tm && c.tn == p_tn && c.bk == p_bk &&  \
-        c.blk == 32)                                                                               \
-        return dtype == DT_BF16 ? launch_bf16_one<p_wm, p_wn, p_tm, p_tn, p_bk, MODE_BF16>(a, st)  \
-                                : launch_bf16_one<p_wm, p_wn, p_tm, p_tn, p_bk, MODE_X3>(a, st);
+    if (!mfma_bf16_has(dtype, c)) {
+        set_error("mfma bf16: configuration not instantiated for this form");
+        return ST_UNSUPPORTED;
+    }
+#define X(p_wm, p_wn, p_tm, p_tn, p_bk, p_forms)                                                    \
+    if (c.waves_m == p_wm && c.waves_n == p_wn && c.tm == p_tm && c.tn == p_tn && c.bk == p_bk) {   \
+        if ((p_forms & 1) && dtype == DT_BF16)                                                      \
+            return launch_bf16_one<p_wm, p_wn, p_tm, p_tn, p_bk, MODE_BF16>(a, st);                 \
+        if ((p_forms & 2) && dtype == DT_BF16X3)                                                    \
+            return launch_bf16_one<p_wm, p_wn, p_tm, p_tn, p_bk, MODE_X3>(a, st);                   \
+    }
     GAT_BF16_CFGS(X)
 #undef X
     set_error("mfma bf16: configuration not instantiated");

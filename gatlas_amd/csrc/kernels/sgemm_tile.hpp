@@ -77,22 +77,26 @@ struct TileIO {
             if (!TRANS) {
                 const int r = idx / (BK / 4), kq = (idx % (BK / 4)) * 4;
                 const int gr = r0 + r, gk = k0 + kq;
-                if (full || (vec && gr < rows_lim && gk + 4 <= k_lim)) {
-                    reg[q] = *(const f4v*)(src + (int64_t)gr * ld + gk);
+                const float* p = src + (int64_t)gr * ld + gk;
+                if (full) {  // uniform: interior tiles
+                    reg[q] = *(const f4v*)p;
+                } else if (vec && gr < rows_lim && gk + 4 <= k_lim) {
+                    reg[q] = *(const f4v*)p;
                 } else {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        reg[q][e] = (gr < rows_lim && gk + e < k_lim) ? src[(int64_t)gr * ld + gk + e] : 0.0f;
+                    for (int e = 0; e < 4; ++e) reg[q][e] = (gr < rows_lim && gk + e < k_lim) ? p[e] : 0.0f;
                 }
             } else {
                 const int k = idx % BK, rq = (idx / BK) * 4;
                 const int gk = k0 + k, gr = r0 + rq;
-                if (full || (vec && gk < k_lim && gr + 4 <= rows_lim)) {
-                    reg[q] = *(const f4v*)(src + (int64_t)gk * ld + gr);
+                const float* p = src + (int64_t)gk * ld + gr;
+                if (full) {
+                    reg[q] = *(const f4v*)p;
+                } else if (vec && gk < k_lim && gr + 4 <= rows_lim) {
+                    reg[q] = *(const f4v*)p;
                 } else {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        reg[q][e] = (gk < k_lim && gr + e < rows_lim) ? src[(int64_t)gk * ld + gr + e] : 0.0f;
+                    for (int e = 0; e < 4; ++e) reg[q][e] = (gk < k_lim && gr + e < rows_lim) ? p[e] : 0.0f;
                 }
             }
         }
