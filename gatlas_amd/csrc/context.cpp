@@ -174,6 +174,8 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
     if (dtype == DT_F64) return MfmaCfg{2, 2, 2, 2, 16, 16};
     if (dtype == DT_FP8) return kFp8Cfg;
     if (dtype == DT_BF16 || dtype == DT_BF16X3) {
+        // 256 x 256 tiles of 8 waves once there are enough of them to fill the chip
+        if (mn >= 4096) return dtype == DT_BF16 ? MfmaCfg{2, 4, 4, 2, 64, 32} : MfmaCfg{2, 4, 4, 2, 16, 32};
         if (mn >= 1024) return MfmaCfg{2, 2, 2, 2, 32, 32};
         if (mn >= 256) return MfmaCfg{2, 2, 1, 1, 64, 32};
         return MfmaCfg{1, 1, 2, 2, 64, 32};

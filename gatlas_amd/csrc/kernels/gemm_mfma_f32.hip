@@ -31,7 +31,12 @@ __global__ __launch_bounds__(WM* WN * 64) void sgemm_mfma_kernel(
     X(2, 2, 1, 1, 32, 32)    \
     X(1, 1, 2, 2, 32, 32)    \
     X(2, 2, 2, 2, 32, 16)    \
-    X(2, 2, 4, 4, 16, 16)
+    X(2, 2, 4, 4, 16, 16)    \
+    X(2, 2, 4, 4, 8, 32)     \
+    X(2, 4, 4, 2, 16, 32)    \
+    X(2, 4, 4, 2, 32, 32)    \
+    X(4, 2, 2, 4, 16, 32)    \
+    X(4, 4, 2, 2, 16, 32)
 
 static const MfmaCfg kF32Cfgs[] = {
 #define X(wm, wn, tm, tn, bk, blk) {wm, wn, tm, tn, bk, blk},

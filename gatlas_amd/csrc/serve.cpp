@@ -378,7 +378,7 @@ int Server::run(gat_serve_stats* out) {
                     ++done;
                 }
                 events_free_.push_back(L.ev);
-                if (batched) slots_[L.slot].busy = false;
+                if (L.slot >= 0) slots_[L.slot].busy = false;
                 inflight[s].pop_front();
                 progress = true;
             }
@@ -407,6 +407,22 @@ int Server::run(gat_serve_stats* out) {
         if (batched) {
             for (int kind = 1; kind >= 0; --kind) {  // GEMV first: short, latency-bound
                 if (pend[kind].empty()) continue;
+                if (pend[kind].size() == 1) {
+                    // a batch of one is the single-problem kernel itself (same
+                    // tile routine, same bits), on the least-loaded stream
+                    size_t sb = 0;
+                    for (size_t i = 1; i < streams_.size(); ++i)
+                        if (inflight[i].size() < inflight[sb].size()) sb = i;
+                    Launch L;
+                    int rc = launch_one(pend[kind][0], (int)sb, L);
+                    if (rc) return rc;
+                    pend[kind].clear();
+                    inflight[sb].push_back(std::move(L));
+                    ++launches;
+                    ++launched_reqs;
+                    progress = true;
+                    continue;
+                }
                 // a free slot on an idle stream first (the batch overlaps the
                 // running ones), else any free slot (it queues on its stream)
                 int s = -1;
