@@ -131,16 +131,13 @@ void Context::clear_tuned() {
     tuned_.clear();
 }
 
-// The FP8 kernel has one fixed shape: 2 x 2 waves of 2 x 2 32x32 blocks, K 128.
-static const MfmaCfg kFp8Cfg{2, 2, 2, 2, 128, 32};
-
 int Context::num_cfgs(int dtype) {
     switch (dtype) {
         case DT_F32: return mfma_f32_num_cfgs();
         case DT_F64: return mfma_f64_num_cfgs();
         case DT_BF16:
         case DT_BF16X3: return mfma_bf16_num_cfgs(dtype);
-        case DT_FP8: return 1;
+        case DT_FP8: return mfma_fp8_num_cfgs();
     }
     return 0;
 }
@@ -151,7 +148,7 @@ MfmaCfg Context::cfg_at(int dtype, int i) {
         case DT_F64: return mfma_f64_cfg(i);
         case DT_BF16:
         case DT_BF16X3: return mfma_bf16_cfg(dtype, i);
-        case DT_FP8: return kFp8Cfg;
+        case DT_FP8: return mfma_fp8_cfg(i);
     }
     return MfmaCfg{0, 0, 0, 0, 0, 0};
 }
@@ -162,9 +159,7 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
         case DT_F64: return mfma_f64_has(c);
         case DT_BF16:
         case DT_BF16X3: return mfma_bf16_has(dtype, c);
-        case DT_FP8:
-            return c.waves_m == kFp8Cfg.waves_m && c.waves_n == kFp8Cfg.waves_n && c.tm == kFp8Cfg.tm &&
-                   c.tn == kFp8Cfg.tn && c.bk == kFp8Cfg.bk && c.blk == kFp8Cfg.blk;
+        case DT_FP8: return mfma_fp8_has(c);
     }
     return false;
 }
@@ -172,7 +167,7 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
 MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
     const int64_t mn = a.M < a.N ? a.M : a.N;
     if (dtype == DT_F64) return MfmaCfg{2, 2, 2, 2, 16, 16};
-    if (dtype == DT_FP8) return kFp8Cfg;
+    if (dtype == DT_FP8) return mn >= 4096 ? MfmaCfg{2, 4, 2, 2, 128, 32} : MfmaCfg{2, 2, 2, 2, 128, 32};
     if (dtype == DT_BF16 || dtype == DT_BF16X3) {
         // 256 x 256 tiles of 8 waves once there are enough of them to fill the chip
         if (mn >= 4096) return dtype == DT_BF16 ? MfmaCfg{2, 4, 4, 2, 64, 32} : MfmaCfg{2, 4, 4, 2, 16, 32};
@@ -198,14 +193,14 @@ int Context::gemm(int dtype, const GemmArgs& a, const MfmaCfg* cfg) {
         case DT_F64: return launch_gemm_mfma_f64(c, a, stream);
         case DT_BF16:
         case DT_BF16X3: return launch_gemm_mfma_bf16(dtype, c, a, stream);
-        case DT_FP8: return gemm_fp8(a);
+        case DT_FP8: return gemm_fp8(a, c);
     }
     set_error("gemm: precision form not available");
     return ST_UNSUPPORTED;
 }
 
-int Context::gemm_fp8(const GemmArgs& a) {
-    if (a.scaleA && a.scaleB) return launch_gemm_fp8(a, stream);
+int Context::gemm_fp8(const GemmArgs& a, const MfmaCfg& c) {
+    if (a.scaleA && a.scaleB) return launch_gemm_fp8(c, a, stream);
     // FP32 operands: quantise op(A) and op(B)^T into the workspace first.
     if (a.batch != 1) {
         set_error("fp8 gemm from FP32 operands: batch must be 1");
@@ -237,7 +232,7 @@ int Context::gemm_fp8(const GemmArgs& a) {
     q.ldb = a.K;
     q.scaleA = sA;
     q.scaleB = sB;
-    return launch_gemm_fp8(q, stream);
+    return launch_gemm_fp8(c, q, stream);
 }
 
 }  // namespace gat

@@ -64,7 +64,7 @@ public:
     static bool has_cfg(int dtype, const MfmaCfg& c);
 
 private:
-    int gemm_fp8(const GemmArgs& a);
+    int gemm_fp8(const GemmArgs& a, const MfmaCfg& c);
     std::unordered_map<void*, size_t> owned_;
     void* work_ = nullptr;
     size_t work_bytes_ = 0;

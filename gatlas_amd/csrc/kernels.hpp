@@ -113,6 +113,9 @@ int launch_convert_f32_bf16(const float* x, void* y, size_t n, hipStream_t st);
 // and the scaled GEMM (A8 = op(A) as M x K, B8 = op(B)^T as N x K).
 int launch_quantize_fp8(const float* src, int64_t rs, int64_t cs, int rows, int cols, void* out, float* scales,
                         hipStream_t st);
-int launch_gemm_fp8(const GemmArgs& a, hipStream_t st);
+int launch_gemm_fp8(const MfmaCfg& cfg, const GemmArgs& a, hipStream_t st);
+int mfma_fp8_num_cfgs();
+MfmaCfg mfma_fp8_cfg(int i);
+bool mfma_fp8_has(const MfmaCfg& cfg);
 
 }  // namespace gat

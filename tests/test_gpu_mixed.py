@@ -16,6 +16,7 @@ import pytest
 import torch
 
 from gatlas_amd import _capi
+from gatlas_amd.device import Context
 from oracle import oracle as ora
 from tests.helpers import bf16_bits_to_f32, dequant_fp8, op, uniform_host
 
@@ -155,6 +156,29 @@ def test_fp8_exact_product_of_dequantised(ctx, ta, tb, M, N, K):
     cwo, nwo, _ = ora.error_metrics(got, C64o, MAGo)
     _say("fp8 vs original FP32 inputs: componentwise %.3e normwise %.3e" % (cwo, nwo))
     assert cwo <= 0.25
+
+
+@pytest.mark.parametrize("cfg", Context.tile_cfgs(_capi.GAT_FP8), ids=str)
+@pytest.mark.parametrize("M,N,K", [(300, 520, 384), (512, 256, 200)])
+def test_fp8_every_cfg(ctx, cfg, M, N, K):
+    """Every FP8 tile shape (through gat_gemm_cfg from FP32 operands, ragged
+    edges, K not a multiple of the 128-deep scale block) against the FP64
+    product of the dequantised operands."""
+    _say("fp8 cfg", cfg, M, N, K)
+    A, B, C0 = _operands(M, N, K, 0, 0, 35)
+    dA, dB, dC = ctx.to_device(A), ctx.to_device(B), ctx.to_device(C0)
+    kb = (K + 127) // 128
+    A8, B8 = ctx.empty(M * K, np.uint8), ctx.empty(N * K, np.uint8)
+    sA, sB = ctx.empty(((M + 127) // 128) * kb, np.float32), ctx.empty(((N + 127) // 128) * kb, np.float32)
+    ctx.quantize_fp8(dA, M, K, K, False, A8, sA)
+    ctx.quantize_fp8(dB, N, K, N, True, B8, sB)
+    ctx.gemm(dA, dB, dC, M, N, K, alpha=1.5, beta=-0.5, dtype=_capi.GAT_FP8, cfg=cfg)
+    Ad = dequant_fp8(A8.download(), sA.download(), M, K)
+    Bd = dequant_fp8(B8.download(), sB.download(), N, K)
+    C64 = 1.5 * Ad @ Bd.T - 0.5 * C0.reshape(M, N).astype(np.float64)
+    MAG = 1.5 * np.abs(Ad) @ np.abs(Bd).T + 0.5 * np.abs(C0.reshape(M, N))
+    cw, _, _ = ora.error_metrics(dC.download(), C64, MAG)
+    assert cw <= TOL, (cfg, cw)
 
 
 @pytest.mark.parametrize("dt", [_capi.GAT_BF16X3, _capi.GAT_FP8, _capi.GAT_BF16])
