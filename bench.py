@@ -322,6 +322,7 @@ def main():
         "data": "synthetic (U[-1,1] from the device generator, seed 0x6A7A5EED)",
         "config": {"workload": "SGEMM C = 1.5*A*B - 0.5*C, M=N=K=%d, FP32 row-major, no transpose" % n,
                    "M": n, "N": n, "K": n, "kernel": "MFMA FP32 (v_mfma_f32_32x32x2f32)",
+                   "tile(wm,wn,tm,tn,bk,blk)": list(ctx.tile_for(n, n, n)),
                    "pct_peak_fp32": round(100.0 * tflops / PEAK_FP32, 2),
                    "wall_s_timed_region": round(wall, 4)},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",

@@ -175,7 +175,9 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
         if (mn >= 256) return MfmaCfg{2, 2, 1, 1, 64, 32};
         return MfmaCfg{1, 1, 2, 2, 64, 32};
     }
-    // FP32 (and the FP32-output forms): large square-ish -> 128x128 tiles.
+    // FP32: 256 x 256 tiles of 16 waves (2 x 2 blocks each) once they fill
+    // the chip, else 128 x 128 tiles.
+    if (mn >= 4096) return MfmaCfg{4, 4, 2, 2, 16, 32};
     if (mn >= 1024) return MfmaCfg{2, 2, 2, 2, 32, 32};
     if (mn >= 256) return MfmaCfg{2, 2, 1, 1, 32, 32};
     return MfmaCfg{1, 1, 2, 2, 32, 32};

@@ -34,7 +34,7 @@ int GroupedPlan::choose_shape(const Context& ctx, int dtype, const GroupedDesc& 
 }
 
 int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* d, size_t n,
-                              std::vector<GroupedProblemDev>& probs, std::vector<int4>& work,
+                              std::vector<GroupedProblemDev>& probs, std::vector<int4>& work, GroupedQueues& q,
                               std::vector<int>& shapes, double& flops, const int* force_shapes) {
     if (dtype != DT_F32 && dtype != DT_BF16) {
         set_error("grouped gemm: dtype must be f32 or bf16");
@@ -44,11 +44,13 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
     probs.resize(n);
     shapes.assign(n, 0);
     flops = 0;
-    struct Tile {
-        double cost;
-        int p, m0, n0;
+    // A unit is a band of tile rows of one problem: a problem of few tiles is
+    // one unit, a larger one is cut into up to GP_XCDS bands.
+    struct Unit {
+        double cost, tcost;
+        int p, T, r0, r1, tn;
     };
-    std::vector<Tile> tl;
+    std::vector<Unit> units;
     for (size_t i = 0; i < n; ++i) {
         const GroupedDesc& g = d[i];
         if (g.M < 0 || g.N < 0 || g.K < 0 || g.M > INT32_MAX || g.N > INT32_MAX || g.K > INT32_MAX ||
@@ -76,15 +78,42 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
         p.alpha = (float)g.alpha;
         p.beta = (float)g.beta;
         const int T = s == 0 ? 128 : 64;
-        for (int64_t m0 = 0; m0 < g.M; m0 += T)
-            for (int64_t n0 = 0; n0 < g.N; n0 += T)
-                tl.push_back(Tile{(double)T * T * (double)(g.K + 32), (int)i, (int)m0, (int)n0});
+        const int tm = (int)((g.M + T - 1) / T), tn = (int)((g.N + T - 1) / T);
+        const double tc = (double)T * T * (double)(g.K + 32);  // cost model of one tile
+        const int nb = (tm * tn <= 32) ? 1 : std::min(GP_XCDS, tm);
+        for (int b = 0; b < nb && tm > 0 && tn > 0; ++b) {
+            const int r0 = b * tm / nb, r1 = (b + 1) * tm / nb;
+            if (r1 > r0) units.push_back(Unit{(double)(r1 - r0) * tn * tc, tc, (int)i, T, r0, r1, tn});
+        }
         flops += 2.0 * (double)g.M * (double)g.N * (double)g.K;
     }
-    // longest tiles first; ties keep problem order
-    std::stable_sort(tl.begin(), tl.end(), [](const Tile& a, const Tile& b) { return a.cost > b.cost; });
-    work.resize(tl.size());
-    for (size_t i = 0; i < tl.size(); ++i) work[i] = make_int4(tl[i].p, tl[i].m0, tl[i].n0, 0);
+    // Units onto XCD queues, longest first onto the least-loaded queue, so
+    // each problem's operand panels stay in one XCD's L2 where possible.
+    std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& b) { return a.cost > b.cost; });
+    struct Tile {
+        double cost;
+        int p, m0, n0;
+    };
+    std::vector<Tile> lists[GP_XCDS];
+    double load[GP_XCDS] = {0};
+    for (const Unit& u : units) {
+        int x = 0;
+        for (int y = 1; y < GP_XCDS; ++y)
+            if (load[y] < load[x]) x = y;
+        load[x] += u.cost;
+        // column-major inside the band: consecutive tiles share a B panel
+        for (int c = 0; c < u.tn; ++c)
+            for (int r = u.r0; r < u.r1; ++r) lists[x].push_back(Tile{u.tcost, u.p, r * u.T, c * u.T});
+    }
+    work.clear();
+    for (int x = 0; x < GP_XCDS; ++x) {
+        // within a queue the longest tiles start first; ties keep unit order
+        std::stable_sort(lists[x].begin(), lists[x].end(),
+                         [](const Tile& a, const Tile& b) { return a.cost > b.cost; });
+        q.off[x] = (int)work.size();
+        for (const Tile& t : lists[x]) work.push_back(make_int4(t.p, t.m0, t.n0, 0));
+    }
+    q.off[GP_XCDS] = (int)work.size();
     return ST_OK;
 }
 
@@ -92,7 +121,7 @@ int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n,
                         const int* force_shapes) {
     std::vector<GroupedProblemDev> probs;
     std::vector<int4> work;
-    int rc = build_tables(ctx, dtype, d, n, probs, work, shapes_, flops_, force_shapes);
+    int rc = build_tables(ctx, dtype, d, n, probs, work, queues_, shapes_, flops_, force_shapes);
     if (rc) return rc;
     ctx_ = &ctx;
     bf16_ = dtype == DT_BF16;
@@ -102,9 +131,9 @@ int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n,
     const size_t wbytes = std::max<size_t>(1, work.size()) * sizeof(int4);
     GAT_HIP_TRY(hipMalloc(&d_probs_, pbytes));
     GAT_HIP_TRY(hipMalloc(&d_work_, wbytes));
-    GAT_HIP_TRY(hipMalloc((void**)&d_counter_, 2 * sizeof(int)));
-    // the kernel leaves its counter pair at zero; it starts there once
-    GAT_HIP_TRY(hipMemsetAsync(d_counter_, 0, 2 * sizeof(int), ctx.stream));
+    GAT_HIP_TRY(hipMalloc((void**)&d_counter_, (GP_XCDS + 1) * sizeof(int)));
+    // the kernel leaves its counters at zero; they start there once
+    GAT_HIP_TRY(hipMemsetAsync(d_counter_, 0, (GP_XCDS + 1) * sizeof(int), ctx.stream));
     if (!probs.empty())
         GAT_HIP_TRY(hipMemcpyAsync(d_probs_, probs.data(), probs.size() * sizeof(GroupedProblemDev),
                                    hipMemcpyHostToDevice, ctx.stream));
@@ -118,7 +147,7 @@ int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n,
         rc = grouped_gemm_prepare(bf16_);  // attribute calls stay outside the capture
         if (rc) return rc;
         GAT_HIP_TRY(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
-        rc = launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, (int)nwork_,
+        rc = launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, queues_,
                                  d_counter_, grid_, ctx.stream);
         hipError_t e = hipStreamEndCapture(ctx.stream, &graph_);
         if (rc) return rc;
@@ -134,8 +163,8 @@ int GroupedPlan::run() {
         GAT_HIP_TRY(hipGraphLaunch(exec_, ctx_->stream));
         return ST_OK;
     }
-    return launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, (int)nwork_,
-                               d_counter_, grid_, ctx_->stream);
+    return launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, queues_, d_counter_,
+                               grid_, ctx_->stream);
 }
 
 }  // namespace gat

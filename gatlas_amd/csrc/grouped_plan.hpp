@@ -44,7 +44,7 @@ public:
     // Device descriptors and the cost-ordered (problem, m0, n0) tile list
     // for n problems; also the tile shape per problem and the total flops.
     static int build_tables(const Context& ctx, int dtype, const GroupedDesc* d, size_t n,
-                            std::vector<GroupedProblemDev>& probs, std::vector<int4>& work,
+                            std::vector<GroupedProblemDev>& probs, std::vector<int4>& work, GroupedQueues& q,
                             std::vector<int>& shapes, double& flops, const int* force_shapes = nullptr);
 
 private:
@@ -54,6 +54,7 @@ private:
     void* d_work_ = nullptr;
     int* d_counter_ = nullptr;
     size_t nwork_ = 0;
+    GroupedQueues queues_{};
     int grid_ = 0;
     double flops_ = 0;
     std::vector<int> shapes_;

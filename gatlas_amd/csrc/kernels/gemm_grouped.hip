@@ -1,17 +1,20 @@
 // Persistent grouped GEMM: many independent problems of different shapes
 // and transposes in one launch.
 //
-// The host builds a work list of (problem, m0, n0) tiles sorted by
-// decreasing tile cost (tile area x K), so the longest tiles start first.
+// The host splits the (problem, m0, n0) tiles into one queue per XCD: each
+// problem, or each band of tile rows of a large one, goes whole to one queue
+// (longest first onto the least-loaded queue), so its operand panels are
+// read into one XCD's L2; inside a queue the longest tiles start first.
 // A grid of about two work-groups per CU loops: one work-item takes the
-// next list index from a device counter (atomicAdd), the group computes that
-// tile with the tile routine of the problem's precision, shape and transpose
-// pair, and every group leaves the loop once the list is exhausted.
-// counter[0] is the list cursor and counter[1] counts the groups that have
-// left; the last group to leave (every other group has made its final fetch
-// by then) sets both back to zero, so consecutive launches -- and replays of
-// a captured graph holding this one kernel node -- start from a zeroed pair.
-// The pair must be zero before the first launch.
+// next index of its XCD's queue from that queue's counter (atomicAdd), and
+// once the home queue is drained, from the other queues in turn; the group
+// computes the tile with the routine of the problem's precision, shape and
+// transpose pair, and leaves when every queue is drained.
+// counter[0..7] are the queue cursors and counter[8] counts the groups that
+// have left; the last group to leave (every other group has made its final
+// fetches by then) sets all of them back to zero, so consecutive launches --
+// and replays of a captured graph holding this one kernel node -- start
+// from zeroed counters.  They must be zero before the first launch.
 //
 // Tile shapes: 0 = 128 x 128 (2 x 2 waves of 2 x 2 32x32 blocks),
 //              1 = 64 x 64 (2 x 2 waves of one 32x32 block).
@@ -46,16 +49,30 @@ __device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int 
 
 template <int BF16>
 __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupedProblemDev* __restrict__ probs,
-                                                           const int4* __restrict__ work, int nwork,
+                                                           const int4* __restrict__ work, GroupedQueues q,
                                                            int* __restrict__ counter) {
     extern __shared__ float gsmem[];
     __shared__ int next;
+    const int home = (int)(blockIdx.x % GP_XCDS);
+    int qi = 0;  // queues tried so far: the home queue first, then the others in turn
     for (;;) {
         __syncthreads();  // every work-item has read the previous `next`
-        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) {
+            int w = -1;
+            for (; qi < GP_XCDS; ++qi) {
+                const int x = (home + qi) % GP_XCDS;
+                if (q.off[x] == q.off[x + 1]) continue;
+                const int i = atomicAdd(&counter[x], 1);
+                if (q.off[x] + i < q.off[x + 1]) {
+                    w = q.off[x] + i;
+                    break;
+                }
+            }
+            next = w;
+        }
         __syncthreads();
         const int w = next;
-        if (w >= nwork) break;  // uniform across the group
+        if (w < 0) break;  // uniform across the group: every queue is drained
         const int4 t = work[w];
         const GroupedProblemDev p = probs[t.x];
         const int shape = (p.flags >> GP_SHAPE_SHIFT) & 15;
@@ -75,9 +92,8 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupedProblemD
     }
     if (threadIdx.x == 0) {
         __threadfence();
-        if (atomicAdd(&counter[1], 1) == (int)gridDim.x - 1) {
-            atomicExch(&counter[0], 0);
-            atomicExch(&counter[1], 0);
+        if (atomicAdd(&counter[GP_XCDS], 1) == (int)gridDim.x - 1) {
+            for (int x = 0; x <= GP_XCDS; ++x) atomicExch(&counter[x], 0);
         }
     }
 }
@@ -98,15 +114,15 @@ int grouped_gemm_prepare(int bf16) {
     return ST_OK;
 }
 
-int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* work, int nwork, int* counter,
-                        int grid, hipStream_t st) {
+int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* work, const GroupedQueues& q,
+                        int* counter, int grid, hipStream_t st) {
     const size_t smem = grouped_smem_bytes(bf16);
     const int rc = grouped_gemm_prepare(bf16);
     if (rc) return rc;
     if (bf16)
-        hipLaunchKernelGGL(grouped_gemm_kernel<1>, dim3(grid), dim3(256), smem, st, probs, work, nwork, counter);
+        hipLaunchKernelGGL(grouped_gemm_kernel<1>, dim3(grid), dim3(256), smem, st, probs, work, q, counter);
     else
-        hipLaunchKernelGGL(grouped_gemm_kernel<0>, dim3(grid), dim3(256), smem, st, probs, work, nwork, counter);
+        hipLaunchKernelGGL(grouped_gemm_kernel<0>, dim3(grid), dim3(256), smem, st, probs, work, q, counter);
     GAT_HIP_TRY(hipGetLastError());
     return ST_OK;
 }

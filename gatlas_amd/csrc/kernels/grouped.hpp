@@ -22,13 +22,22 @@ struct GroupedProblemDev {
     float alpha, beta;
 };
 
+// The tile list is split into one queue per XCD (work-group i of the
+// persistent grid runs on XCD i % 8): queue x holds entries
+// [off[x], off[x+1]) of the list.
+constexpr int GP_XCDS = 8;
+struct GroupedQueues {
+    int off[GP_XCDS + 1];
+};
+
 size_t grouped_smem_bytes(int bf16);
 // Sets the kernel's dynamic LDS limit (once); needed before graph capture.
 int grouped_gemm_prepare(int bf16);
-// `work` holds nwork int4 entries {problem, m0, n0, 0}; `counter` points to
-// two ints that are zero before the first launch and zero again after every
-// launch completes.
-int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* work, int nwork, int* counter,
-                        int grid, hipStream_t st);
+// `work` holds int4 entries {problem, m0, n0, 0} in per-XCD queues `q`;
+// `counter` points to GP_XCDS + 1 ints (one cursor per queue, one count of
+// finished groups) that are zero before the first launch and zero again
+// after every launch completes.
+int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* work, const GroupedQueues& q,
+                        int* counter, int grid, hipStream_t st);
 
 }  // namespace gat

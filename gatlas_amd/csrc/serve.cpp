@@ -214,8 +214,8 @@ int Server::create(Context& ctx, const gat_serve_cfg& cfg) {
             b.stream = (int)(i % (size_t)cfg_.streams);
             GAT_HIP_TRY(hipHostMalloc((void**)&b.h_stage, stage_bytes_));
             GAT_HIP_TRY(hipMalloc((void**)&b.d_stage, stage_bytes_));
-            GAT_HIP_TRY(hipMalloc((void**)&b.d_counter, 2 * sizeof(int)));
-            GAT_HIP_TRY(hipMemsetAsync(b.d_counter, 0, 2 * sizeof(int), ctx.stream));
+            GAT_HIP_TRY(hipMalloc((void**)&b.d_counter, (GP_XCDS + 1) * sizeof(int)));
+            GAT_HIP_TRY(hipMemsetAsync(b.d_counter, 0, (GP_XCDS + 1) * sizeof(int), ctx.stream));
         }
     }
     GAT_HIP_TRY(hipStreamSynchronize(ctx.stream));
@@ -294,7 +294,8 @@ int Server::launch_batch(int kind, std::vector<int>& pend, int slot, Launch& L) 
         std::vector<int4> work;
         std::vector<int> used;
         double flops = 0;
-        int rc = GroupedPlan::build_tables(*ctx_, DT_F32, descs.data(), descs.size(), probs, work, used, flops,
+        GroupedQueues q{};
+        int rc = GroupedPlan::build_tables(*ctx_, DT_F32, descs.data(), descs.size(), probs, work, q, used, flops,
                                            shapes.data());
         if (rc) return rc;
         // [descriptors | tile list] in one staging buffer, one copy
@@ -305,8 +306,8 @@ int Server::launch_batch(int kind, std::vector<int>& pend, int slot, Launch& L) 
         GAT_HIP_TRY(hipMemcpyAsync(b.d_stage, b.h_stage, woff + work.size() * sizeof(int4), hipMemcpyHostToDevice,
                                    st));
         const int grid = (int)std::max<size_t>(1, std::min<size_t>(2 * (size_t)std::max(ctx_->num_cu, 1), work.size()));
-        rc = launch_grouped_gemm(0, (const GroupedProblemDev*)b.d_stage, (const int4*)(b.d_stage + woff),
-                                 (int)work.size(), b.d_counter, grid, st);
+        rc = launch_grouped_gemm(0, (const GroupedProblemDev*)b.d_stage, (const int4*)(b.d_stage + woff), q,
+                                 b.d_counter, grid, st);
         if (rc) return rc;
         pend.erase(pend.begin(), pend.begin() + take);
     } else {

@@ -210,6 +210,12 @@ class Context:
                                      int(verbose), ctypes.byref(c), ctypes.byref(g)), "gat_tune_gemm")
         return c.as_tuple(), g.value
 
+    def tile_for(self, M, N, K, transA=False, transB=False, dtype=_capi.GAT_F32):
+        """The tile configuration gat_gemm uses for this shape (tuned or default)."""
+        c = _capi.TileCfg()
+        self.lib.gat_tuned_get(self.h, dtype, int(transA), int(transB), M, N, K, ctypes.byref(c))
+        return c.as_tuple()
+
     def tuned_clear(self):
         check(self.lib.gat_tuned_clear(self.h), "gat_tuned_clear")
 

@@ -12,6 +12,8 @@ Workloads (inputs U[-1,1] from the device generator):
   fp8_8192     resident e4m3 operands with block scales, C = A*B
   sgemv16384   y = A*x + y, FP32, 16384 x 16384 (A^T with sgemv16384t)
   saxpy        z = 1.5*x + y, FP32, n = 2^28
+  grouped512   512 FP32 problems (M, N, K log-uniform in [64, 4096]) in one
+               persistent grouped launch (untuned tile rule)
 """
 import argparse
 import sys
@@ -63,6 +65,25 @@ def run(ctx, name, reps):
             ctx.fill_uniform(a, 5, i)
         for _ in range(reps):
             ctx.gemv(A, x, y, m, m, transA=name.endswith("t"), alpha=1.0, beta=1.0)
+    elif name == "grouped512":
+        # 512 problems, M, N, K log-uniform in [64, 4096] (multiples of 16),
+        # random transposes, FP32; one persistent launch per repetition
+        rng = np.random.default_rng(0x67525550)
+        v = np.clip(np.round(np.exp(rng.uniform(np.log(64), np.log(4096), (512, 3))) / 16) * 16, 64,
+                    4096).astype(int)
+        tr = rng.integers(0, 2, (512, 2))
+        probs = []
+        for i in range(512):
+            M, N, K = (int(x) for x in v[i])
+            dA, dB, dC = ctx.empty(M * K), ctx.empty(K * N), ctx.empty(M * N)
+            ctx.fill_uniform(dA, 21, 2 * i)
+            ctx.fill_uniform(dB, 21, 2 * i + 1)
+            probs.append(dict(transA=int(tr[i, 0]), transB=int(tr[i, 1]), M=M, N=N, K=K, A=dA, B=dB, C=dC))
+        g = ctx.grouped(probs, _capi.GAT_F32, use_graph=False)
+        for _ in range(reps):
+            g.run()
+        ctx.sync()
+        g.close()
     elif name == "saxpy":
         ns = 1 << 28
         X, Y, Z = ctx.empty(ns), ctx.empty(ns), ctx.empty(ns)
@@ -76,14 +97,47 @@ def run(ctx, name, reps):
     print("[kernel_runs] %s x%d done" % (name, reps), flush=True)
 
 
+def sweep_tiles(ctx, n, forms, reps):
+    """Every instantiated tile configuration of each form at n^3 (C = A*B, no
+    transpose): one warm-up, then `reps` launches timed with HIP events."""
+    codes = {"f32": _capi.GAT_F32, "f64": _capi.GAT_F64, "bf16": _capi.GAT_BF16, "bf16x3": _capi.GAT_BF16X3,
+             "fp8": _capi.GAT_FP8}
+    print("| form | tile | ms | TFLOP/s |\n|---|---|---|---|", flush=True)
+    for form in forms:
+        dt = codes[form]
+        f = ctx.empty(n * n, np.float64 if dt == _capi.GAT_F64 else np.float32)
+        g = ctx.empty(n * n, f.dtype)
+        C = ctx.empty(n * n, f.dtype)
+        ctx.fill_uniform(f, 1, 0)
+        ctx.fill_uniform(g, 1, 1)
+        A, B = f, g
+        if dt == _capi.GAT_BF16:
+            A, B = ctx.empty(n * n, np.uint16), ctx.empty(n * n, np.uint16)
+            ctx.convert_bf16(f, A)
+            ctx.convert_bf16(g, B)
+        for cfg in Context.tile_cfgs(dt):
+            ctx.gemm(A, B, C, n, n, n, dtype=dt, cfg=cfg)
+            ctx.sync()
+            ctx.timer_start()
+            for _ in range(reps):
+                ctx.gemm(A, B, C, n, n, n, dtype=dt, cfg=cfg)
+            ms = ctx.timer_stop() / reps
+            print("| %s | %s | %.3f | %.1f |" % (form, cfg, ms, 2.0 * n ** 3 / ms / 1e9), flush=True)
+
+
 def main(argv=None):
     p = argparse.ArgumentParser()
     p.add_argument("--reps", type=int, default=3)
-    p.add_argument("workloads", nargs="+")
+    p.add_argument("--sweep-tiles", type=int, default=0, metavar="N",
+                   help="time every tile configuration of the given forms at N^3")
+    p.add_argument("workloads", nargs="+", help="workload names, or forms with --sweep-tiles")
     a = p.parse_args(argv)
     ctx = Context(0)
-    for w in a.workloads:
-        run(ctx, w, a.reps)
+    if a.sweep_tiles:
+        sweep_tiles(ctx, a.sweep_tiles, a.workloads, a.reps)
+    else:
+        for w in a.workloads:
+            run(ctx, w, a.reps)
     ctx.close()
     return 0
 
