@@ -11,6 +11,7 @@
 // Compat (reference geometry, KernelMatvecBuffer.hpp:180-325): work-item g
 //   owns rows g*bh .. g*bh+bh-1, each a j-ascending chain of separately
 //   rounded multiply and add -- the arithmetic of the reference's host check.
+#include "gemv_grouped.hpp"
 #include "gemv_rows.hpp"
 
 namespace gat {
@@ -153,7 +154,10 @@ static int launch_native_t(int transA, const GemvCfg& cfg, size_t M, size_t N, T
 int launch_gemv_native(int dtype, int transA, size_t M, size_t N, double alpha, const void* A, size_t lda,
                        const void* x, double beta, void* y, void* work, hipStream_t st, const GemvCfg* cfg) {
     if (M == 0) return ST_OK;
-    const GemvCfg c = cfg ? *cfg : GemvCfg{256, 4, 2};
+    // default launch shape: the fastest of the 16384^2 sweep (128 work-items,
+    // 2 rows per wave, one load per row in flight); the grouped kernel uses
+    // the same rows/unroll so results match bit for bit
+    const GemvCfg c = cfg ? *cfg : GemvCfg{128, GEMV_GROUPED_ROWS, GEMV_GROUPED_UNROLL};
     if (dtype == DT_F32)
         return launch_native_t<float, f4v>(transA, c, M, N, (float)alpha, (const float*)A, lda, (const float*)x,
                                            (float)beta, (float*)y, work, st);

@@ -3,9 +3,9 @@
 //
 // Problem p owns the work-groups [blk0_p, blk0_{p+1}); a work-group finds
 // its problem by binary search over the ascending blk0 table, then runs
-// the same wave routine with the same launch shape (256 work-items, 4 rows
-// per wave, 2 loads in flight) as the single-problem kernel's default, so
-// each problem's result is bit-identical to its single launch.
+// the same wave routine with the same rows per wave and loads in flight as
+// the single-problem kernel's default, so each problem's result is
+// bit-identical to its single launch.
 #include "gemv_grouped.hpp"
 #include "gemv_rows.hpp"
 

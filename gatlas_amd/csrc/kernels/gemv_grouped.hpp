@@ -5,10 +5,11 @@
 
 namespace gat {
 
-// Launch shape of the grouped kernel: 256 work-items per group, ROWS rows
-// per wave, UNROLL loads in flight -- the single-problem default
-// GemvCfg{256, GEMV_GROUPED_ROWS, GEMV_GROUPED_UNROLL}.
-enum : int { GEMV_GROUPED_ROWS = 4, GEMV_GROUPED_UNROLL = 2 };
+// Per-wave shape of the grouped kernel (256 work-items per group): ROWS rows
+// per wave, UNROLL loads in flight per row -- the rows/unroll of the
+// single-problem default, so a problem's arithmetic is the same in both
+// (the work-group size does not enter it).
+enum : int { GEMV_GROUPED_ROWS = 2, GEMV_GROUPED_UNROLL = 1 };
 
 struct GemvProblemDev {
     const float* A;

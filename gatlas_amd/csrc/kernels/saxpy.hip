@@ -109,7 +109,9 @@ static int launch_native_t(const AxpyCfg& c, size_t n, T alpha, const T* x, cons
 int launch_saxpy_native(int dtype, size_t n, double alpha, const void* x, const void* y, void* z,
                         hipStream_t st, const AxpyCfg* cfg) {
     if (n == 0) return ST_OK;
-    const AxpyCfg c = cfg ? *cfg : AxpyCfg{256, 4, 1};
+    // default: the fastest of the 2^28 sweep (one 16-byte vector per operand
+    // per work-item and step, non-temporal)
+    const AxpyCfg c = cfg ? *cfg : AxpyCfg{256, 1, 1};
     if (dtype == DT_F32)
         return launch_native_t<float, f4v>(c, n, (float)alpha, (const float*)x, (const float*)y, (float*)z, st);
     if (dtype == DT_F64)

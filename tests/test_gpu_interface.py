@@ -106,3 +106,23 @@ def test_autotuner_installs_bucket_choice(ctx, tmp_path):
     assert ctx.lib.gat_tuned_get(ctx.h, _capi.GAT_F32, 0, 0, 2048, 2048, 2048, ctypes.byref(got)) == 1
     assert got.as_tuple() == best.as_tuple()
     print("[tuned] 2048^3 ->", best.as_tuple(), "%.1f GFLOPS" % gf.value, flush=True)
+
+
+def _point_usec(tmp_path, name, extra):
+    j = str(tmp_path / name)
+    rc, out, err = _run([os.path.join(BIN, "bench_matmul"), "-d", "gpu", "-j", j, "-T", "floatmfma", "-n", "2048",
+                         "-g", "2", "-y", "2", "-x", "2", "-t", "3"] + extra)
+    assert rc == 0 and "***DONE***" in out, err
+    lines = [ln for ln in out.splitlines() if ln.startswith("[0] ") and " usec" in ln]
+    assert lines, out
+    return float(lines[-1].split()[1])
+
+
+def test_transfer_inclusive_timing(tmp_path):
+    # -s times the host-to-device upload of A, B (and C) with the launch,
+    # -r the device-to-host read of C (GatlasBenchmark.cpp:198-265): 48 MiB
+    # over PCIe costs far more than a 2048^3 matrix-core GEMM (~0.15 ms)
+    plain = _point_usec(tmp_path, "p.jnl", [])
+    both = _point_usec(tmp_path, "sr.jnl", ["-s", "-r"])
+    print("[timing] device-only %.0f us, with transfers %.0f us" % (plain, both), flush=True)
+    assert both > plain + 500
