@@ -175,6 +175,34 @@ def grouped_bench(ctx, dtype, reps=10):
             "loop_ms_p99": round(l99, 3), "speedup": round(l50 / g50, 3)}
 
 
+def sgemm_1024_bench(ctx, with_cpu):
+    """The reference's own runnable case: SGEMM C = A*B, M = N = K = 1024,
+    FP32, no transpose, on the GPU and, timed in the same run, the reference
+    buffer-kernel arithmetic ported to the host cores (whole product)."""
+    import numpy as np
+
+    n = 1024
+    A, B, C = ctx.empty(n * n), ctx.empty(n * n), ctx.empty(n * n)
+    ctx.fill_uniform(A, 0x6A7A5EED, 0)
+    ctx.fill_uniform(B, 0x6A7A5EED, 1)
+    ms, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), 20, 3)
+    out = {"gpu_ms": round(ms, 4), "gpu_TFLOP/s": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 2),
+           "tile(wm,wn,tm,tn,bk,blk)": list(ctx.tile_for(n, n, n))}
+    for a in (A, B, C):
+        a.free()
+    if with_cpu:
+        from oracle import oracle as ora
+        from tests.helpers import uniform_host
+        hA, hB = uniform_host(n * n, 0x6A7A5EED, 0), uniform_host(n * n, 0x6A7A5EED, 1)
+        hC = np.zeros(n * n, np.float32)
+        t0 = time.perf_counter()
+        ora.port_matmul_rows(hA, hB, hC, n, n, n, 0, n)
+        t = time.perf_counter() - t0
+        out.update({"cpu_port_s": round(t, 4), "cpu_port_TFLOP/s": round(2.0 * n ** 3 / t / 1e12, 4),
+                    "cpu_threads": ora.num_threads(), "gpu_over_cpu": round(t / (ms * 1e-3), 1)})
+    return out
+
+
 def sweep_bench(ctx, sizes=(1024, 2048, 4096, 8192)):
     """General form C = 1.5*op(A)*op(B) - 0.5*C, all four transpose pairs,
     square sizes, FP32 and FP64 (default tile choice): TFLOP/s per case."""
@@ -396,6 +424,7 @@ def main():
         sec["dgemm_8192"] = {"ms": round(ms_d, 3), "TFLOP/s": round(tf, 2), "frac_fp64": round(tf / PEAK_FP64, 4)}
         for a in (Ad, Bd, Cd):
             a.free()
+        sec["sgemm_1024"] = sgemm_1024_bench(ctx, not args.no_cpu)
         sec["general_form_sweep"] = sweep_bench(ctx)
         sec["tuner"] = tuner_bench(ctx)
     if not args.no_grouped:

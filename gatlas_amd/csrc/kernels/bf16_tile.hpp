@@ -63,15 +63,15 @@ struct StageIO<MODE_BF16, ROWS, BK, NT, TRANS> {
             const __bf16* p = !TRANS ? src + (int64_t)gr * ld + gk : src + (int64_t)gk * ld + gr;
             const bool vin = !TRANS ? (gr < rlim && gk + W <= klim) : (gk < klim && gr + W <= rlim);
             if (full) {  // uniform: interior tiles
-                reg[q] = *(const u32x4*)p;
+                reg[q] = gload<u32x4>(p);
             } else if (vec && vin) {
-                reg[q] = *(const u32x4*)p;
+                reg[q] = gload<u32x4>(p);
             } else {
                 __bf16 t[W];
 #pragma unroll
                 for (int e = 0; e < W; ++e) {
                     const bool in = !TRANS ? (gr < rlim && gk + e < klim) : (gk < klim && gr + e < rlim);
-                    t[e] = in ? p[e] : (__bf16)0.0f;
+                    t[e] = in ? gload<__bf16>(p + e) : (__bf16)0.0f;
                 }
                 reg[q] = *(const u32x4*)t;
             }
@@ -120,14 +120,14 @@ struct StageIO<MODE_X3, ROWS, BK, NT, TRANS> {
             const float* p = !TRANS ? src + (int64_t)gr * ld + gk : src + (int64_t)gk * ld + gr;
             const bool vin = !TRANS ? (gr < rlim && gk + W <= klim) : (gk < klim && gr + W <= rlim);
             if (full) {  // uniform: interior tiles
-                reg[q] = *(const f4v*)p;
+                reg[q] = gload<f4v>(p);
             } else if (vec && vin) {
-                reg[q] = *(const f4v*)p;
+                reg[q] = gload<f4v>(p);
             } else {
 #pragma unroll
                 for (int e = 0; e < W; ++e) {
                     const bool in = !TRANS ? (gr < rlim && gk + e < klim) : (gk < klim && gr + e < rlim);
-                    reg[q][e] = in ? p[e] : 0.0f;
+                    reg[q][e] = in ? gload<float>(p + e) : 0.0f;
                 }
             }
         }
@@ -195,7 +195,7 @@ __device__ __forceinline__ void bf16_tile(const void* __restrict__ Av, const voi
     IoA ioa;
     IoB iob;
     const int KT = (K + BK - 1) / BK;
-    auto gload = [&](int t) {
+    auto fetch = [&](int t) {
         const int k0 = t * BK;
         const bool full = full_mn && (k0 + BK <= K);
         ioa.load(A, lda, m0, k0, M, K, full, vec_ok, tid);
@@ -206,7 +206,7 @@ __device__ __forceinline__ void bf16_tile(const void* __restrict__ Av, const voi
         iob.store(buf + PLANE_A, buf + 2 * PLANE_A + PLANE_B, tid);
     };
 
-    gload(0);
+    fetch(0);
     sstore(smem);
     __syncthreads();
 
@@ -219,7 +219,7 @@ __device__ __forceinline__ void bf16_tile(const void* __restrict__ Av, const voi
         const __bf16* Bh = Ah + PLANE_A;
         const __bf16* Al = Ah + PLANE_A + PLANE_B;
         const __bf16* Bl = Al + PLANE_A;
-        if (t + 1 < KT) gload(t + 1);
+        if (t + 1 < KT) fetch(t + 1);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             bf16x8 ah[TM], bh[TN];
@@ -263,8 +263,8 @@ __device__ __forceinline__ void bf16_tile(const void* __restrict__ Av, const voi
                 if (row < M && col < N) {
                     float* p = C + (int64_t)row * ldc + col;
                     float v = alpha * acc[i][j][r];
-                    if (beta != 0.0f) v += beta * *p;
-                    *p = v;
+                    if (beta != 0.0f) v += beta * gload<float>(p);
+                    gstore(p, v);
                 }
             }
 }

@@ -11,6 +11,7 @@
 #pragma once
 
 #include "../kernels.hpp"
+#include "gmem.hpp"
 
 namespace gat {
 
@@ -38,10 +39,10 @@ __device__ __forceinline__ void gemv_rows_wave(int64_t M, int64_t N, T alpha, co
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
                 const int64_t cc = c + u * step + lane * W;
-                xv[u] = *(const V*)(x + cc);
+                xv[u] = gload<V>(x + cc);
 #pragma unroll
                 for (int r = 0; r < ROWS; ++r)
-                    av[u][r] = __builtin_nontemporal_load((const V*)(A + (r0 + r) * lda + cc));
+                    av[u][r] = gload_nt<V>(A + (r0 + r) * lda + cc);
             }
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u)
@@ -53,10 +54,10 @@ __device__ __forceinline__ void gemv_rows_wave(int64_t M, int64_t N, T alpha, co
     }
     // remainder (and unaligned / partial-row-block cases): scalar columns
     for (int64_t cc = c + lane; cc < N; cc += 64) {
-        const T xv = x[cc];
+        const T xv = gload<T>(x + cc);
 #pragma unroll
         for (int r = 0; r < ROWS; ++r)
-            if (r0 + r < M) acc[r] += A[(r0 + r) * lda + cc] * xv;
+            if (r0 + r < M) acc[r] += gload<T>(A + (r0 + r) * lda + cc) * xv;
     }
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) acc[r] = wave_sum<T>(acc[r]);
@@ -66,8 +67,8 @@ __device__ __forceinline__ void gemv_rows_wave(int64_t M, int64_t N, T alpha, co
         for (int r = 1; r < ROWS; ++r)
             if (lane == r) s = acc[r];
         T v = alpha * s;
-        if (beta != (T)0) v += beta * y[r0 + lane];
-        y[r0 + lane] = v;
+        if (beta != (T)0) v += beta * gload<T>(y + r0 + lane);
+        gstore(y + r0 + lane, v);
     }
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "../kernels.hpp"
+#include "gmem.hpp"
 
 namespace gat {
 

@@ -79,24 +79,24 @@ struct TileIO {
                 const int gr = r0 + r, gk = k0 + kq;
                 const float* p = src + (int64_t)gr * ld + gk;
                 if (full) {  // uniform: interior tiles
-                    reg[q] = *(const f4v*)p;
+                    reg[q] = gload<f4v>(p);
                 } else if (vec && gr < rows_lim && gk + 4 <= k_lim) {
-                    reg[q] = *(const f4v*)p;
+                    reg[q] = gload<f4v>(p);
                 } else {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) reg[q][e] = (gr < rows_lim && gk + e < k_lim) ? p[e] : 0.0f;
+                    for (int e = 0; e < 4; ++e) reg[q][e] = (gr < rows_lim && gk + e < k_lim) ? gload<float>(p + e) : 0.0f;
                 }
             } else {
                 const int k = idx % BK, rq = (idx / BK) * 4;
                 const int gk = k0 + k, gr = r0 + rq;
                 const float* p = src + (int64_t)gk * ld + gr;
                 if (full) {
-                    reg[q] = *(const f4v*)p;
+                    reg[q] = gload<f4v>(p);
                 } else if (vec && gk < k_lim && gr + 4 <= rows_lim) {
-                    reg[q] = *(const f4v*)p;
+                    reg[q] = gload<f4v>(p);
                 } else {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) reg[q][e] = (gk < k_lim && gr + e < rows_lim) ? p[e] : 0.0f;
+                    for (int e = 0; e < 4; ++e) reg[q][e] = (gk < k_lim && gr + e < rows_lim) ? gload<float>(p + e) : 0.0f;
                 }
             }
         }
@@ -155,14 +155,14 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
 
     f4v ra[IoA::NV], rb[IoB::NV];
     const int KT = (K + BK - 1) / BK;
-    auto gload = [&](int t) {
+    auto fetch = [&](int t) {
         const int k0 = t * BK;
         const bool full = full_mn && (k0 + BK <= K);
         IoA::load(ra, A, lda, m0, k0, M, K, full, vec_ok, tid);
         IoB::load(rb, B, ldb, n0, k0, N, K, full, vec_ok, tid);
     };
 
-    gload(0);
+    fetch(0);
     IoA::store(ra, smem, tid);
     IoB::store(rb, smem + BM * LDK, tid);
     __syncthreads();
@@ -174,7 +174,7 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
     for (int t = 0; t < KT; ++t) {
         const float* As = smem + (t & 1) * (BM + BN) * LDK;
         const float* Bs = As + BM * LDK;
-        if (t + 1 < KT) gload(t + 1);
+        if (t + 1 < KT) fetch(t + 1);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += Op::KGROUP) {
             f4v a[TM], b[TN];
@@ -208,8 +208,8 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
                 if (row < M && col < N) {
                     float* p = C + (int64_t)row * ldc + col;
                     float v = alpha * acc[i][j][r];
-                    if (beta != 0.0f) v += beta * *p;
-                    *p = v;
+                    if (beta != 0.0f) v += beta * gload<float>(p);
+                    gstore(p, v);
                 }
             }
 }
