@@ -47,12 +47,22 @@ __device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int 
                                                      p.beta, m0, n0, vec, smem);
 }
 
+// LDS of the tile buffers (the larger of the two shapes); the group's next
+// work index follows them.  The kernel declares no static LDS, so the
+// dynamic block -- and every 16-byte tile access in it -- starts aligned.
+template <int BF16>
+constexpr size_t grouped_tile_bytes() {
+    constexpr size_t a = BF16 ? Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM : SgemmShape<2, 2, 2, 2, 32, 32>::SMEM;
+    constexpr size_t b = BF16 ? Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM : SgemmShape<2, 2, 1, 1, 32, 32>::SMEM;
+    return ((a > b ? a : b) + 15) / 16 * 16;
+}
+
 template <int BF16>
 __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupedProblemDev* __restrict__ probs,
                                                            const int4* __restrict__ work, GroupedQueues q,
                                                            int* __restrict__ counter) {
-    extern __shared__ float gsmem[];
-    __shared__ int next;
+    extern __shared__ __attribute__((aligned(16))) float gsmem[];
+    int& next = *(int*)((char*)gsmem + grouped_tile_bytes<BF16>());
     const int home = (int)(blockIdx.x % GP_XCDS);
     int qi = 0;  // queues tried so far: the home queue first, then the others in turn
     for (;;) {
@@ -71,8 +81,11 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupedProblemD
             next = w;
         }
         __syncthreads();
-        const int w = next;
-        if (w < 0) break;  // uniform across the group: every queue is drained
+        // the same value in every lane: read it as a scalar so the problem
+        // fields live in scalar registers and the tile routines' in-bounds
+        // tests are uniform branches
+        const int w = __builtin_amdgcn_readfirstlane(next);
+        if (w < 0) break;  // every queue is drained
         const int4 t = work[w];
         const GroupedProblemDev p = probs[t.x];
         const int shape = (p.flags >> GP_SHAPE_SHIFT) & 15;
@@ -99,12 +112,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupedProblemD
 }
 
 size_t grouped_smem_bytes(int bf16) {
-    if (!bf16) {
-        const size_t a = SgemmShape<2, 2, 2, 2, 32, 32>::SMEM, b = SgemmShape<2, 2, 1, 1, 32, 32>::SMEM;
-        return a > b ? a : b;
-    }
-    const size_t a = Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM, b = Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM;
-    return a > b ? a : b;
+    return (bf16 ? grouped_tile_bytes<1>() : grouped_tile_bytes<0>()) + 16;
 }
 
 int grouped_gemm_prepare(int bf16) {
