@@ -185,7 +185,7 @@ def sgemm_1024_bench(ctx, with_cpu):
     A, B, C = ctx.empty(n * n), ctx.empty(n * n), ctx.empty(n * n)
     ctx.fill_uniform(A, 0x6A7A5EED, 0)
     ctx.fill_uniform(B, 0x6A7A5EED, 1)
-    ms, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), 20, 3)
+    ms, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), 200, 5)
     out = {"gpu_ms": round(ms, 4), "gpu_TFLOP/s": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 2),
            "tile(wm,wn,tm,tn,bk,blk)": list(ctx.tile_for(n, n, n))}
     for a in (A, B, C):
@@ -240,11 +240,13 @@ def tuner_bench(ctx, sizes=(1024, 2048, 4096)):
             A, B, C = ctx.empty(n * n), ctx.empty(n * n), ctx.empty(n * n)
             for i, a in enumerate((A, B, C)):
                 ctx.fill_uniform(a, 0x6A7A5EED, i)
-            ms0, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), 10, 2)
+            # about 5 ms of launches at 100 TFLOP/s, 10..200 of them
+            steps = max(10, min(200, int(5e-3 * 100e12 / (2.0 * n ** 3))))
+            ms0, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), steps, 5)
             t0 = time.perf_counter()
             cfg, _ = ctx.tune_gemm(n, n, n, journal=os.path.join(d, "tune_%d.journal" % n))
             tune_s = time.perf_counter() - t0
-            ms1, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), 10, 2)
+            ms1, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), steps, 5)
             out["sgemm_%d" % n] = {"default_TFLOP/s": round(2.0 * n ** 3 / (ms0 * 1e-3) / 1e12, 2),
                                    "tuned_TFLOP/s": round(2.0 * n ** 3 / (ms1 * 1e-3) / 1e12, 2),
                                    "tile(wm,wn,tm,tn,bk,blk)": list(cfg), "search_s": round(tune_s, 2)}

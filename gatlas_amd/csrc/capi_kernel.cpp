@@ -1,5 +1,7 @@
 // extern "C" layer of the parameter interface, the journal, the search and
 // the autotuner.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <iostream>
 #include <memory>
@@ -325,23 +327,19 @@ int gat_tune_gemm(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, siz
     QuietReports quiet(verbose == 0);
     Journal journal(journal_path);
     Bench bench(app, *k, journal, verbose != 0);
+    // about 3 ms of work per timed trial at an assumed 100 TFLOP/s, 1..64 launches
+    const double flops = 2.0 * (double)M * (double)N * (double)K;
+    bench.setLaunchesPerTrial((size_t)std::max(1.0, std::min(64.0, std::ceil(3e-3 * 100e12 / std::max(flops, 1.0)))));
     LoopOpts o;
-    const size_t total = m->totalVariations();
-    size_t seedXp = 0;
+    // The instantiated tile table is small, so the first step times every
+    // entry (tile shape and BK/BLK variant alike); the second re-ranks the
+    // winner's variants from the journal and confirms the fixed point.
     auto enumerate = [&](int step, const std::vector<size_t>& b) {
-        if (step == 0)
-            return getParamsMatmulMfma(*m, false, M, N, K, transA, transB, std::vector<size_t>(),
-                                       (long)(b.empty() ? seedXp : b[11]));
+        if (step == 0) return getParamsMatmulMfma(*m, false, M, N, K, transA, transB, std::vector<size_t>(), FREE);
         return getParamsMatmulMfma(*m, false, M, N, K, transA, transB, std::vector<size_t>{b[7], b[8], b[9], b[10]},
                                    FREE);
     };
-    auto bump = [&](std::vector<size_t>& b) {
-        if (!b.empty()) {
-            b[11] = (b[11] + 1) % total;
-            return true;
-        }
-        return ++seedXp < total;
-    };
+    auto bump = [](std::vector<size_t>&) { return false; };
     const std::vector<size_t> best = emSearch(*k, bench, journal, o, {7, 8, 9, 10, 11}, enumerate, bump);
     if (best.empty()) return GAT_ERR_UNSUPPORTED;
     m->setParams(best);

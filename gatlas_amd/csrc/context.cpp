@@ -164,23 +164,40 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
     return false;
 }
 
+// Untuned tile rule: the largest tile of the form that still gives at least
+// one work-group per CU of an MI355X (256 CUs), counting batched problems;
+// a problem too small to fill the chip with any tile gets the 64 x 64 tile
+// of four waves (one wave when a side is below 256).  At 1024^3, 2048^3 and
+// 4096^3 FP32 this picks the same tiles as an exhaustive timing of the table.
 MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
-    const int64_t mn = a.M < a.N ? a.M : a.N;
     if (dtype == DT_F64) return MfmaCfg{2, 2, 2, 2, 16, 16};
-    if (dtype == DT_FP8) return mn >= 4096 ? MfmaCfg{2, 4, 2, 2, 128, 32} : MfmaCfg{2, 2, 2, 2, 128, 32};
-    if (dtype == DT_BF16 || dtype == DT_BF16X3) {
-        // 256 x 256 tiles of 8 waves once there are enough of them to fill the chip
-        if (mn >= 4096) return dtype == DT_BF16 ? MfmaCfg{2, 4, 4, 2, 64, 32} : MfmaCfg{2, 4, 4, 2, 16, 32};
-        if (mn >= 1024) return MfmaCfg{2, 2, 2, 2, 32, 32};
-        if (mn >= 256) return MfmaCfg{2, 2, 1, 1, 64, 32};
-        return MfmaCfg{1, 1, 2, 2, 64, 32};
+    constexpr int64_t kFillTiles = 256;
+    const int64_t batch = a.batch > 1 ? a.batch : 1;
+    auto fills = [&](const MfmaCfg& c) {
+        const int64_t bm = (int64_t)c.waves_m * c.tm * c.blk, bn = (int64_t)c.waves_n * c.tn * c.blk;
+        return ((a.M + bm - 1) / bm) * ((a.N + bn - 1) / bn) * batch >= kFillTiles;
+    };
+    const int64_t mn = a.M < a.N ? a.M : a.N;
+    if (dtype == DT_FP8) {
+        const MfmaCfg wide{2, 4, 2, 2, 128, 32};  // 128 x 256, 8 waves
+        return fills(wide) ? wide : MfmaCfg{2, 2, 2, 2, 128, 32};
     }
-    // FP32: 256 x 256 tiles of 16 waves (2 x 2 blocks each) once they fill
-    // the chip, else 128 x 128 tiles.
-    if (mn >= 4096) return MfmaCfg{4, 4, 2, 2, 16, 32};
-    if (mn >= 1024) return MfmaCfg{2, 2, 2, 2, 32, 32};
-    if (mn >= 256) return MfmaCfg{2, 2, 1, 1, 32, 32};
-    return MfmaCfg{1, 1, 2, 2, 32, 32};
+    MfmaCfg big, mid, small, tiny;
+    if (dtype == DT_BF16 || dtype == DT_BF16X3) {
+        big = dtype == DT_BF16 ? MfmaCfg{2, 4, 4, 2, 64, 32} : MfmaCfg{2, 4, 4, 2, 16, 32};  // 256 x 256, 8 waves
+        mid = MfmaCfg{2, 2, 2, 2, 32, 32};
+        small = MfmaCfg{2, 2, 1, 1, 64, 32};
+        tiny = MfmaCfg{1, 1, 2, 2, 64, 32};
+    } else {
+        big = MfmaCfg{4, 4, 2, 2, 16, 32};  // 256 x 256, 16 waves
+        mid = MfmaCfg{2, 2, 2, 2, 32, 32};
+        small = MfmaCfg{2, 2, 1, 1, 32, 32};
+        tiny = MfmaCfg{1, 1, 2, 2, 32, 32};
+    }
+    if (fills(big)) return big;
+    if (fills(mid)) return mid;
+    if (fills(small) || mn >= 256) return small;
+    return tiny;
 }
 
 int Context::gemm(int dtype, const GemmArgs& a, const MfmaCfg* cfg) {

@@ -98,7 +98,7 @@ size_t Bench::run(size_t numTrials, const std::vector<size_t>& args, bool toDevi
 
     const auto t0 = std::chrono::steady_clock::now();
     if (toDevice && !k_.setArgs(app_, toDevice)) return 0;
-    for (size_t i = 0; i < numTrials; ++i) {
+    for (size_t i = 0; i < numTrials * launches_; ++i) {
         if (!k_.enqueue(app_)) {
             if (print_) std::cerr << "error: enqueue kernel " << i << std::endl;
             app_.wait();
@@ -118,7 +118,7 @@ size_t Bench::run(size_t numTrials, const std::vector<size_t>& args, bool toDevi
         if (print_) std::cerr << "error: read output data from device" << std::endl;
         return 0;
     }
-    size_t us = (size_t)std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count();
+    size_t us = (size_t)std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count() / launches_;
     if (us == 0) us = 1;  // 0 is reserved for "failed check"
 
     const bool isOk = k_.checkOutput(app_, printDebug);

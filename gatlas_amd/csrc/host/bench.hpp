@@ -59,12 +59,18 @@ public:
     // Optional hook called between the BUILD_OK and RUN_IN_PROGRESS journal
     // records (used by failure-injection tests of the crash/hang recovery).
     std::function<void(const char* stage)> stageHook;
+    // Each of the numTrials launches of run() is repeated `n` times back to
+    // back and the elapsed time divided by n (default 1: one launch per
+    // trial), so kernels of a few tens of microseconds are timed over enough
+    // work for the host clock and launch overhead not to decide the ranking.
+    void setLaunchesPerTrial(size_t n) { launches_ = n ? n : 1; }
 
 private:
     DeviceApp& app_;
     KernelInterface& k_;
     Journal* journal_;
     bool print_;
+    size_t launches_ = 1;
 };
 
 // Report stream of the benchmark loops: standard output, as the command-line

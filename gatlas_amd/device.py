@@ -203,8 +203,9 @@ class Context:
 
     def tune_gemm(self, M, N, K, transA=False, transB=False, dtype=_capi.GAT_F32, journal="gat_tune.journal",
                   verbose=False):
-        """EM search over the MFMA tile table for one shape (journaled); the
-        winner is installed for the shape's size bucket.  Returns (cfg, GFLOP/s)."""
+        """Times every MFMA tile-table entry for one shape, then confirms the
+        winner (EM fixed point), journaled; the winner is installed for the
+        shape's size bucket.  Returns (cfg, GFLOP/s)."""
         c, g = _capi.TileCfg(), ctypes.c_double()
         check(self.lib.gat_tune_gemm(self.h, dtype, int(transA), int(transB), M, N, K, journal.encode(),
                                      int(verbose), ctypes.byref(c), ctypes.byref(g)), "gat_tune_gemm")

@@ -388,8 +388,10 @@ int gat_search_em(gat_kernel *k, const char *journal_path, size_t max_workgroup,
                   int transA, int transB, size_t ntrials, int paranoid, size_t *best, size_t nbest);
 
 /* Autotuner (no reference counterpart; replaces the exhaustive OpenCL search for
- * the matrix-core kernels): EM search over the instantiated tile table for one
- * shape, journaled to journal_path, result installed for the shape's size bucket. */
+ * the matrix-core kernels): times every entry of the instantiated tile table for
+ * one shape (each trial chains enough launches for about 3 ms of work), then
+ * confirms the winner as the EM search's fixed point; journaled to journal_path,
+ * result installed for the shape's size bucket. */
 int gat_tune_gemm(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, size_t N, size_t K,
                   const char *journal_path, int verbose, gat_tile_cfg *best, double *gflops);
 /* Install / query the tile choice of a size bucket (get returns 1 if tuned, 0 default). */

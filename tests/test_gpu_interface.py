@@ -108,6 +108,34 @@ def test_autotuner_installs_bucket_choice(ctx, tmp_path):
     print("[tuned] 2048^3 ->", best.as_tuple(), "%.1f GFLOPS" % gf.value, flush=True)
 
 
+def test_autotuner_not_slower_than_default(ctx, tmp_path):
+    # The default tile is in the searched table, so the installed choice,
+    # timed with device events over 30 launches, is within noise of it or
+    # faster.
+    n = 1024
+    A, B, C = ctx.empty(n * n), ctx.empty(n * n), ctx.empty(n * n)
+    for i, a in enumerate((A, B, C)):
+        ctx.fill_uniform(a, 77, i)
+
+    def ms():
+        ctx.gemm(A, B, C, n, n, n)
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(30):
+            ctx.gemm(A, B, C, n, n, n)
+        return ctx.timer_stop() / 30
+
+    ctx.tuned_clear()
+    try:
+        t0 = ms()
+        cfg, _ = ctx.tune_gemm(n, n, n, journal=str(tmp_path / "t1024.jnl"))
+        t1 = ms()
+    finally:
+        ctx.tuned_clear()
+    print("[tuned] 1024^3 default %.4f ms, tuned %s %.4f ms" % (t0, cfg, t1), flush=True)
+    assert t1 <= 1.10 * t0
+
+
 def _point_usec(tmp_path, name, extra):
     j = str(tmp_path / name)
     rc, out, err = _run([os.path.join(BIN, "bench_matmul"), "-d", "gpu", "-j", j, "-T", "floatmfma", "-n", "2048",
