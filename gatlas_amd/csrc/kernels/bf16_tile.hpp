@@ -31,6 +31,42 @@ __device__ __forceinline__ void split_bf16(float a, __bf16& hi, __bf16& lo) {
 // Global -> register -> LDS staging of a ROWS x BK tile for one operand.
 //   direct (k-contiguous source): element (r, k) at src[(r0 + r) * ld + k0 + k]
 //   transposed (row-contiguous):  element (r, k) at src[(k0 + k) * ld + r0 + r]
+// A transposed tile is staged in blocks of KB consecutive k by W consecutive
+// rows (KB 16-byte loads, one per k); the store transposes a block in
+// registers and writes each of its W rows as one KB-element vector, so LDS
+// takes 4-, 8- or 16-byte stores instead of one store per element.  Lanes
+// take consecutive k-blocks first: the KB loads of a wavefront read whole
+// row segments, and one store instruction spreads over all LDS banks.
+// KB is the largest of 8, 4, 2 that divides the per-thread vector count
+// and BK (1 -- one store per element -- when none does).
+template <int NV, int BK>
+struct TransBlock {
+    static constexpr int KB = (NV % 8 == 0 && BK % 8 == 0)   ? 8
+                              : (NV % 4 == 0 && BK % 4 == 0) ? 4
+                              : (NV % 2 == 0 && BK % 2 == 0) ? 2
+                                                             : 1;
+};
+
+// bf16 bit patterns of two elements packed into one dword (first in the low half)
+__device__ __forceinline__ unsigned pack_bf16(unsigned lo, unsigned hi) { return (lo & 0xFFFFu) | (hi << 16); }
+
+// Writes KB bf16 values (bit patterns in the low halves of b[]) to dst
+template <int KB>
+__device__ __forceinline__ void store_kvec(__bf16* dst, const unsigned (&b)[KB]) {
+    if constexpr (KB == 1) {
+        *(unsigned short*)dst = (unsigned short)b[0];
+    } else if constexpr (KB == 2) {
+        *(unsigned*)dst = pack_bf16(b[0], b[1]);
+    } else if constexpr (KB == 4) {
+        *(u32x2*)dst = u32x2{pack_bf16(b[0], b[1]), pack_bf16(b[2], b[3])};
+    } else {
+        *(u32x4*)dst = u32x4{pack_bf16(b[0], b[1]), pack_bf16(b[2], b[3]), pack_bf16(b[4], b[5]),
+                             pack_bf16(b[6], b[7])};
+    }
+}
+
+__device__ __forceinline__ unsigned bf16_bits(__bf16 h) { return (unsigned)__builtin_bit_cast(unsigned short, h); }
+
 template <int MODE, int ROWS, int BK, int NT, bool TRANS>
 struct StageIO;
 
@@ -42,7 +78,21 @@ struct StageIO<MODE_BF16, ROWS, BK, NT, TRANS> {
     static constexpr int NV = ROWS * BK / W / NT;
     static_assert(NV >= 1 && NV * W * NT == ROWS * BK, "tile must split into 16-byte vectors per thread");
     static constexpr int LDK = BK + 8;
+    static constexpr int KB = TRANS ? TransBlock<NV, BK>::KB : 1;
     u32x4 reg[NV];
+
+    // element (row, k) offset of vector q of this thread in the tile
+    __device__ static __forceinline__ void coords(int q, int tid, int& r, int& k) {
+        if (!TRANS) {
+            const int idx = q * NT + tid;
+            r = idx / (BK / W);
+            k = (idx % (BK / W)) * W;
+        } else {
+            const int bi = (q / KB) * NT + tid;  // block index; q % KB is the k within the block
+            k = (bi % (BK / KB)) * KB + q % KB;
+            r = (bi / (BK / KB)) * W;
+        }
+    }
 
     // full: the whole tile is in bounds; otherwise each vector is checked on
     // its own (out-of-range elements read as zero, an in-range vector is one
@@ -51,15 +101,9 @@ struct StageIO<MODE_BF16, ROWS, BK, NT, TRANS> {
                                          int klim, bool full, bool vec, int tid) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
-            const int idx = q * NT + tid;
-            int gr, gk;
-            if (!TRANS) {
-                gr = r0 + idx / (BK / W);
-                gk = k0 + (idx % (BK / W)) * W;
-            } else {
-                gk = k0 + idx % BK;
-                gr = r0 + (idx / BK) * W;
-            }
+            int r, k;
+            coords(q, tid, r, k);
+            const int gr = r0 + r, gk = k0 + k;
             const __bf16* p = !TRANS ? src + (int64_t)gr * ld + gk : src + (int64_t)gk * ld + gr;
             const bool vin = !TRANS ? (gr < rlim && gk + W <= klim) : (gk < klim && gr + W <= rlim);
             if (full) {  // uniform: interior tiles
@@ -78,17 +122,25 @@ struct StageIO<MODE_BF16, ROWS, BK, NT, TRANS> {
         }
     }
     __device__ __forceinline__ void store(__bf16* hi, __bf16* /*lo*/, int tid) const {
+        if constexpr (!TRANS) {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            const int idx = q * NT + tid;
-            if (!TRANS) {
-                const int r = idx / (BK / W), k = (idx % (BK / W)) * W;
+            for (int q = 0; q < NV; ++q) {
+                int r, k;
+                coords(q, tid, r, k);
                 *(u32x4*)(hi + r * LDK + k) = reg[q];
-            } else {
-                const int k = idx % BK, r = (idx / BK) * W;
-                const __bf16* v = (const __bf16*)&reg[q];
+            }
+        } else {
 #pragma unroll
-                for (int e = 0; e < W; ++e) hi[(r + e) * LDK + k] = v[e];
+            for (int qb = 0; qb < NV; qb += KB) {
+                int r, k;
+                coords(qb, tid, r, k);
+#pragma unroll
+                for (int e = 0; e < W; ++e) {
+                    unsigned b[KB];
+#pragma unroll
+                    for (int j = 0; j < KB; ++j) b[j] = reg[qb + j][e >> 1] >> (16 * (e & 1));
+                    store_kvec<KB>(hi + (r + e) * LDK + k, b);
+                }
             }
         }
     }
@@ -102,21 +154,28 @@ struct StageIO<MODE_X3, ROWS, BK, NT, TRANS> {
     static constexpr int NV = ROWS * BK / W / NT;
     static_assert(NV >= 1 && NV * W * NT == ROWS * BK, "tile must split into 16-byte vectors per thread");
     static constexpr int LDK = BK + 8;
+    static constexpr int KB = TRANS ? TransBlock<NV, BK>::KB : 1;
     f4v reg[NV];
+
+    __device__ static __forceinline__ void coords(int q, int tid, int& r, int& k) {
+        if (!TRANS) {
+            const int idx = q * NT + tid;
+            r = idx / (BK / W);
+            k = (idx % (BK / W)) * W;
+        } else {
+            const int bi = (q / KB) * NT + tid;
+            k = (bi % (BK / KB)) * KB + q % KB;
+            r = (bi / (BK / KB)) * W;
+        }
+    }
 
     __device__ __forceinline__ void load(const float* __restrict__ src, int64_t ld, int r0, int k0, int rlim,
                                          int klim, bool full, bool vec, int tid) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
-            const int idx = q * NT + tid;
-            int gr, gk;
-            if (!TRANS) {
-                gr = r0 + idx / (BK / W);
-                gk = k0 + (idx % (BK / W)) * W;
-            } else {
-                gk = k0 + idx % BK;
-                gr = r0 + (idx / BK) * W;
-            }
+            int r, k;
+            coords(q, tid, r, k);
+            const int gr = r0 + r, gk = k0 + k;
             const float* p = !TRANS ? src + (int64_t)gr * ld + gk : src + (int64_t)gk * ld + gr;
             const bool vin = !TRANS ? (gr < rlim && gk + W <= klim) : (gk < klim && gr + W <= rlim);
             if (full) {  // uniform: interior tiles
@@ -133,22 +192,34 @@ struct StageIO<MODE_X3, ROWS, BK, NT, TRANS> {
         }
     }
     __device__ __forceinline__ void store(__bf16* hi, __bf16* lo, int tid) const {
+        if constexpr (!TRANS) {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            const int idx = q * NT + tid;
-            __bf16 h[W], l[W];
+            for (int q = 0; q < NV; ++q) {
+                int r, k;
+                coords(q, tid, r, k);
+                __bf16 h[W], l[W];
 #pragma unroll
-            for (int e = 0; e < W; ++e) split_bf16(reg[q][e], h[e], l[e]);
-            if (!TRANS) {
-                const int r = idx / (BK / W), k = (idx % (BK / W)) * W;
+                for (int e = 0; e < W; ++e) split_bf16(reg[q][e], h[e], l[e]);
                 *(u32x2*)(hi + r * LDK + k) = *(const u32x2*)h;
                 *(u32x2*)(lo + r * LDK + k) = *(const u32x2*)l;
-            } else {
-                const int k = idx % BK, r = (idx / BK) * W;
+            }
+        } else {
+#pragma unroll
+            for (int qb = 0; qb < NV; qb += KB) {
+                int r, k;
+                coords(qb, tid, r, k);
 #pragma unroll
                 for (int e = 0; e < W; ++e) {
-                    hi[(r + e) * LDK + k] = h[e];
-                    lo[(r + e) * LDK + k] = l[e];
+                    unsigned bh[KB], bl[KB];
+#pragma unroll
+                    for (int j = 0; j < KB; ++j) {
+                        __bf16 h, l;
+                        split_bf16(reg[qb + j][e], h, l);
+                        bh[j] = bf16_bits(h);
+                        bl[j] = bf16_bits(l);
+                    }
+                    store_kvec<KB>(hi + (r + e) * LDK + k, bh);
+                    store_kvec<KB>(lo + (r + e) * LDK + k, bl);
                 }
             }
         }
@@ -220,6 +291,8 @@ __device__ __forceinline__ void bf16_tile(const void* __restrict__ Av, const voi
         const __bf16* Al = Ah + PLANE_A + PLANE_B;
         const __bf16* Bl = Al + PLANE_A;
         if (t + 1 < KT) fetch(t + 1);
+        // interleave the LDS fragment reads with the matrix instructions
+        __builtin_amdgcn_iglp_opt(0);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             bf16x8 ah[TM], bh[TN];

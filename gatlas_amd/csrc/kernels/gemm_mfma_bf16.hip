@@ -5,8 +5,24 @@
 
 namespace gat {
 
+// Waves per SIMD the kernel asks the register allocator to allow: what the
+// LDS footprint permits (160 KiB per CU), capped by what the 512 registers
+// of a SIMD lane hold at 16 accumulators per 32x32 block plus a working set
+// (fragments, staging, addresses; two operand planes in the x3 form).
+// Without it the allocator may place the accumulators in AGPRs next to a
+// full VGPR set and leave one wave per SIMD where LDS allows two.
+template <int WM, int WN, int TM, int TN, int BK, int MODE>
+constexpr int bf16_waves_per_simd() {
+    constexpr int groups = (int)(163840 / Bf16Shape<WM, WN, TM, TN, BK, MODE>::SMEM);
+    constexpr int by_lds = groups * WM * WN / 4;
+    constexpr int by_regs = 512 / (TM * TN * 16 + (MODE == MODE_X3 ? 160 : 96));
+    constexpr int w = by_lds < by_regs ? by_lds : by_regs;
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
 template <int WM, int WN, int TM, int TN, int BK, int MODE, bool TA, bool TB>
-__global__ __launch_bounds__(WM* WN * 64) void gemm_bf16_kernel(
+__global__ __launch_bounds__(WM* WN * 64)
+__attribute__((amdgpu_waves_per_eu(bf16_waves_per_simd<WM, WN, TM, TN, BK, MODE>()))) void gemm_bf16_kernel(
     const void* __restrict__ Av, const void* __restrict__ Bv, float* __restrict__ C, int M, int N, int K,
     int64_t lda, int64_t ldb, int64_t ldc, int64_t sA, int64_t sB, int64_t sC, float alpha, float beta,
     int tiles_m, int tiles_n, int group_m, int vec_ok) {
@@ -105,10 +121,10 @@ int launch_gemm_mfma_bf16(int dtype, const MfmaCfg& c, const GemmArgs& a, hipStr
     }
 #define X(p_wm, p_wn, p_tm, p_tn, p_bk, p_forms)                                                    \
     if (c.waves_m == p_wm && c.waves_n == p_wn && c.tm == p_tm && c.tn == p_tn && c.bk == p_bk) {   \
-        if ((p_forms & 1) && dtype == DT_BF16)                                                      \
-            return launch_bf16_one<p_wm, p_wn, p_tm, p_tn, p_bk, MODE_BF16>(a, st);                 \
-        if ((p_forms & 2) && dtype == DT_BF16X3)                                                    \
-            return launch_bf16_one<p_wm, p_wn, p_tm, p_tn, p_bk, MODE_X3>(a, st);                   \
+        if constexpr ((p_forms & 1) != 0)                                                           \
+            if (dtype == DT_BF16) return launch_bf16_one<p_wm, p_wn, p_tm, p_tn, p_bk, MODE_BF16>(a, st); \
+        if constexpr ((p_forms & 2) != 0)                                                           \
+            if (dtype == DT_BF16X3) return launch_bf16_one<p_wm, p_wn, p_tm, p_tn, p_bk, MODE_X3>(a, st); \
     }
     GAT_BF16_CFGS(X)
 #undef X
