@@ -1,0 +1,189 @@
+// Direct-to-LDS variant of the BF16 macro tile (BF16 inputs, BK = 64) for
+// interior tiles: operands go from HBM into LDS with gfx950's 16-byte
+// global_load_lds (no register staging, no LDS store instructions), so the
+// next K stage lands in LDS while the MFMAs run on the current one.
+//
+// A global_load_lds of a wavefront writes 1 KiB contiguous in LDS, lane l's
+// 16 bytes at base + 16*l; the layout is chosen by which 16-byte chunk of
+// global memory each lane fetches.
+//   k-contiguous operand (A not transposed, B transposed): [row][k], 128
+//     bytes per row (8 chunks), chunk c of row r stored at position
+//     c ^ (r & 7): the 32 rows x 2 chunks of one fragment read cover every
+//     bank group evenly.
+//   row-contiguous operand (A transposed, B not transposed): [k][row] as in
+//     memory, chunk c of k-row k stored at c ^ (4 * (k & 1)); fragments are
+//     read with ds_read_b64_tr_b16.  Within each 16-lane group, lane i's
+//     half s is element (i % 4) of the 8 bytes addressed by lane 4s + i/4,
+//     so lane j addressing (k-row kb + j/4, row n0 + 4*(j % 4)) hands lane
+//     i rows n0 + i, k-rows kb .. kb+3; two reads make the 8 k values of
+//     an MFMA operand.
+// The MFMA operand layout, accumulators and epilogue are bf16_tile's.
+#pragma once
+
+#include "bf16_tile.hpp"
+
+namespace gat {
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) s16x4 lds_s16x4_t;
+
+// One operand of a K stage: ROWS x 64 bf16, loaded by NW wavefronts.
+template <int ROWS, bool KCONTIG, int NW>
+struct DmaOperand {
+    static constexpr int BYTES = ROWS * 64 * 2;
+    static constexpr int BLOCKS = BYTES / 1024;  // one wavefront instruction each
+    static constexpr int PER_WAVE = BLOCKS / NW;
+    static_assert(PER_WAVE * NW == BLOCKS, "blocks must split evenly over the wavefronts");
+    static constexpr int CPR = ROWS / 8;  // chunks per k-row, row-contiguous layout
+    static constexpr int KPB = 64 / CPR;  // k-rows per block, row-contiguous layout
+    static_assert(KCONTIG || (CPR >= 8 && CPR <= 64 && 64 % CPR == 0), "row-contiguous tile: 64..512 rows");
+
+    __device__ static __forceinline__ int tswz(int k) { return 4 * (k & 1); }
+
+    // Wavefront w fetches its blocks of the stage starting at k0 into `lds`.
+    __device__ static __forceinline__ void issue(const __bf16* src, int64_t ld, int r0, int k0, char* lds, int w,
+                                                 int lane) {
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int b = i * NW + w;
+            const __bf16* g;
+            if constexpr (KCONTIG) {
+                const int row = 8 * b + (lane >> 3);
+                g = src + (int64_t)(r0 + row) * ld + k0 + 8 * ((lane & 7) ^ (row & 7));
+            } else {
+                const int k = b * KPB + lane / CPR;
+                g = src + (int64_t)(k0 + k) * ld + r0 + 8 * ((lane % CPR) ^ tswz(k));
+            }
+            __builtin_amdgcn_global_load_lds((glb_void_t*)g, (lds_void_t*)(lds + b * 1024), 16, 0, 0);
+        }
+    }
+
+    // MFMA operand (8 k values of row rbase + lane%32 at k = kk + 8*(lane/32))
+    __device__ static __forceinline__ bf16x8 frag(const char* lds, int rbase, int kk, int lane) {
+        if constexpr (KCONTIG) {
+            const int row = rbase + (lane & 31);
+            const int c = (kk >> 3) + (lane >> 5);
+            return *(const bf16x8*)(lds + row * 128 + 16 * (c ^ (lane & 7)));
+        } else {
+            const int g = lane >> 4, j = lane & 15;
+            const int n = rbase + 16 * (g & 1) + 4 * (j & 3);
+            const int k = kk + 8 * (g >> 1) + (j >> 2);
+            const int off0 = k * (ROWS * 2) + 16 * ((n >> 3) ^ tswz(k)) + 2 * (n & 7);
+            const int off1 = off0 + 4 * (ROWS * 2);  // k + 4: same parity, same swizzle
+            const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4_t*)(lds + off0));
+            const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4_t*)(lds + off1));
+            const s16x4 v[2] = {lo, hi};
+            return *(const bf16x8*)v;
+        }
+    }
+};
+
+template <int WM, int WN, int TM, int TN>
+struct Bf16DmaShape {
+    static constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    static constexpr size_t SMEM = 2 * (size_t)(BM + BN) * 64 * sizeof(__bf16);
+};
+
+// Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C for
+// a tile wholly inside M x N with K a multiple of 64 and 16-byte aligned
+// rows; all WM*WN*64 threads of the group call it; ends after a barrier.
+template <int WM, int WN, int TM, int TN, bool TA, bool TB>
+__device__ __forceinline__ void bf16_tile_dma(const __bf16* __restrict__ A, const __bf16* __restrict__ B,
+                                              float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb,
+                                              int64_t ldc, float alpha, float beta, int m0, int n0, __bf16* smem) {
+    constexpr int NW = WM * WN;
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    typedef DmaOperand<BM, !TA, NW> OpA;  // op(A) rows are k-contiguous unless A is transposed
+    typedef DmaOperand<BN, TB, NW> OpB;   // op(B)^T rows are k-contiguous when B is transposed
+    constexpr int STAGE = OpA::BYTES + OpB::BYTES;
+    char* const sb = (char*)smem;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = w / WN, wn = w % WN;
+
+    f16v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int KT = K / 64;
+    bf16x8 a0[TM], b0[TN], a1[TM], b1[TN];
+    auto frags = [&](const char* stage, int kk, bf16x8(&a)[TM], bf16x8(&b)[TN]) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = OpA::frag(stage, wm * TM * 32 + i * 32, kk, lane);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = OpB::frag(stage + OpA::BYTES, wn * TN * 32 + j * 32, kk, lane);
+    };
+    auto mma = [&](const bf16x8(&a)[TM], const bf16x8(&b)[TN]) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    };
+    // one MFMA, then one LDS read and one global load if any remain, ...
+    auto interleave = [&]() {
+#pragma unroll
+        for (int i = 0; i < TM * TN; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load (to LDS)
+        }
+    };
+
+    OpA::issue(A, lda, m0, 0, sb, w, lane);
+    OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    frags(sb, 0, a0, b0);
+
+    // Software pipeline over the four 16-deep steps of a stage: the
+    // fragments of step s+1 are read while the MFMAs of step s run; the
+    // wait for the next stage and the barrier come before the last step,
+    // whose MFMAs run under the reads of the next stage's first fragments.
+    // The next stage is fetched unconditionally (the last iteration fetches
+    // its own stage again into the idle buffer), so each half of the
+    // iteration is one scheduling region.
+    for (int t = 0; t < KT; ++t) {
+        const char* cur = sb + (t & 1) * STAGE;
+        char* nxt = sb + ((t + 1) & 1) * STAGE;
+        const int kn = (t + 1 < KT ? t + 1 : t) * 64;
+        OpA::issue(A, lda, m0, kn, nxt, w, lane);
+        OpB::issue(B, ldb, n0, kn, nxt + OpA::BYTES, w, lane);
+        frags(cur, 16, a1, b1);
+        mma(a0, b0);
+        interleave();
+        frags(cur, 32, a0, b0);
+        mma(a1, b1);
+        interleave();
+        frags(cur, 48, a1, b1);
+        mma(a0, b0);
+        interleave();
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
+        __syncthreads();
+        frags(nxt, 0, a0, b0);
+        mma(a1, b1);
+        interleave();
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * TM * 32 + i * 32 + mfma32_row(lane, r);
+                const int col = n0 + wn * TN * 32 + j * 32 + mfma32_col(lane);
+                float* p = C + (int64_t)row * ldc + col;
+                float v = alpha * acc[i][j][r];
+                if (beta != 0.0f) v += beta * gload<float>(p);
+                gstore(p, v);
+            }
+}
+
+}  // namespace gat

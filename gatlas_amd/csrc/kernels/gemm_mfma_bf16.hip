@@ -1,6 +1,7 @@
 // BF16 matrix-core GEMMs on gfx950: one work-group per macro tile in
 // XCD-aware supertile order; the tile computation is bf16_tile
 // (bf16_tile.hpp), shared with the persistent grouped kernel.
+#include "bf16_dma_tile.hpp"
 #include "bf16_tile.hpp"
 
 namespace gat {
@@ -34,6 +35,15 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_simd<WM, WN, TM, TN, BK, MODE>
     C += (int64_t)blockIdx.y * sC;
     int tmi, tni;
     mfma_tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, group_m, tmi, tni);
+    if constexpr (MODE == MODE_BF16 && BK == 64) {
+        // interior tile, whole 64-deep K stages, 16-byte aligned rows: direct-to-LDS staging
+        const int m0 = tmi * S::BM, n0 = tni * S::BN;
+        if (vec_ok && K >= 64 && K % 64 == 0 && m0 + S::BM <= M && n0 + S::BN <= N) {
+            bf16_tile_dma<WM, WN, TM, TN, TA, TB>((const __bf16*)A, (const __bf16*)B, C, M, N, K, lda, ldb, ldc,
+                                                  alpha, beta, m0, n0, smemh);
+            return;
+        }
+    }
     bf16_tile<WM, WN, TM, TN, BK, MODE, TA, TB>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, tmi * S::BM,
                                                tni * S::BN, vec_ok != 0, smemh);
 }
@@ -89,7 +99,9 @@ template <int WM, int WN, int TM, int TN, int BK, int MODE>
 static int launch_bf16_one(const GemmArgs& a, hipStream_t st) {
     constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     constexpr int NPL = MODE == MODE_X3 ? 2 : 1;
-    constexpr size_t smem = 2 * (size_t)NPL * (BM + BN) * (BK + 8) * sizeof(__bf16);
+    constexpr size_t smem_reg = 2 * (size_t)NPL * (BM + BN) * (BK + 8) * sizeof(__bf16);
+    constexpr size_t smem_dma = (MODE == MODE_BF16 && BK == 64) ? Bf16DmaShape<WM, WN, TM, TN>::SMEM : 0;
+    constexpr size_t smem = smem_reg > smem_dma ? smem_reg : smem_dma;
     const int tiles_m = (int)((a.M + BM - 1) / BM), tiles_n = (int)((a.N + BN - 1) / BN);
     const int W = MODE == MODE_X3 ? 4 : 8;  // elements per 16-byte vector
     const int vec_ok = ((uintptr_t)a.A % 16 == 0) && ((uintptr_t)a.B % 16 == 0) && (a.lda % W == 0) &&

@@ -108,6 +108,49 @@ def test_bf16_every_cfg(ctx, dt):
             assert cw <= TOL, (dt, cfg, ta, tb, cw)
 
 
+def _pad_rows(x, rows, cols, extra):
+    """rows x cols row-major array -> rows x (cols + extra), zero padding."""
+    out = np.zeros((rows, cols + extra), dtype=x.dtype)
+    out[:, :cols] = x.reshape(rows, cols)
+    return out
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_bf16_direct_lds_path(ctx, ta, tb):
+    # M, N with whole and partial 256-wide tiles and K a multiple of 64: the
+    # interior tiles of the 64-deep configurations stage operands straight
+    # from HBM into LDS (k-contiguous operands XOR-swizzled, row-contiguous
+    # ones read back with the transposing LDS read), the edge tiles through
+    # registers.  Checked against the FP64 oracle, and bit for bit against
+    # the register path on the same data: leading dimensions one element
+    # longer (not 16-byte multiples) force that path on every tile.
+    M, N, K = 512, 384, 192
+    A, B, C0 = _operands(M, N, K, ta, tb, 35)
+    dA, dB = ctx.to_device(A), ctx.to_device(B)
+    a16, b16 = ctx.empty(A.size, np.uint16), ctx.empty(B.size, np.uint16)
+    ctx.convert_bf16(dA, a16)
+    ctx.convert_bf16(dB, b16)
+    ha, hb = a16.download(), b16.download()
+    ra, ca = (K, M) if ta else (M, K)
+    rb, cb = (N, K) if tb else (K, N)
+    pa, pb = ctx.to_device(_pad_rows(ha, ra, ca, 1)), ctx.to_device(_pad_rows(hb, rb, cb, 1))
+    Ar, Br = bf16_bits_to_f32(ha), bf16_bits_to_f32(hb)
+    C64, MAG = ora.gemm64(Ar, Br, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, C0=C0)
+    for cfg in ctx.tile_cfgs(_capi.GAT_BF16):
+        if cfg[4] != 64:
+            continue
+        _say("bf16 direct-LDS", cfg, ta, tb)
+        d1, d2 = ctx.to_device(C0), ctx.to_device(C0)
+        ctx.gemm(a16, b16, d1, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, dtype=_capi.GAT_BF16,
+                 cfg=cfg)
+        ctx.gemm(pa, pb, d2, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, dtype=_capi.GAT_BF16,
+                 lda=ca + 1, ldb=cb + 1, cfg=cfg)
+        got, ref = d1.download(), d2.download()
+        cw, _, _ = ora.error_metrics(got, C64, MAG)
+        assert cw <= TOL, (cfg, ta, tb, cw)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (cfg, ta, tb)
+
+
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
 def test_bf16_inputs_exact_product(ctx, ta, tb):
     M, N, K = 384, 256, 512
