@@ -179,8 +179,9 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
     };
     const int64_t mn = a.M < a.N ? a.M : a.N;
     if (dtype == DT_FP8) {
-        const MfmaCfg wide{2, 4, 2, 2, 128, 32};  // 128 x 256, 8 waves
-        return fills(wide) ? wide : MfmaCfg{2, 2, 2, 2, 128, 32};
+        const MfmaCfg big{2, 4, 4, 2, 128, 32};  // 256 x 256, 8 waves
+        const MfmaCfg tall{4, 2, 2, 2, 128, 32};  // 256 x 128, 8 waves
+        return fills(big) ? big : fills(tall) ? tall : MfmaCfg{2, 2, 2, 2, 128, 32};
     }
     MfmaCfg big, mid, small, tiny;
     if (dtype == DT_BF16 || dtype == DT_BF16X3) {

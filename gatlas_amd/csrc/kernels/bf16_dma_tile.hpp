@@ -29,38 +29,47 @@ typedef __attribute__((address_space(1))) void glb_void_t;
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) s16x4 lds_s16x4_t;
 
-// One operand of a K stage: ROWS x 64 bf16, loaded by NW wavefronts.
-template <int ROWS, bool KCONTIG, int NW>
+// One operand of a K stage: ROWS rows of 128 bytes of k (64 bf16 or 128
+// fp8 values), loaded by NW wavefronts; T is the element type.  The
+// row-contiguous layout exists for 2-byte elements only.
+template <int ROWS, bool KCONTIG, int NW, typename T = __bf16>
 struct DmaOperand {
-    static constexpr int BYTES = ROWS * 64 * 2;
+    static constexpr int EPC = 16 / (int)sizeof(T);  // elements per 16-byte chunk
+    static constexpr int BYTES = ROWS * 128;
     static constexpr int BLOCKS = BYTES / 1024;  // one wavefront instruction each
     static constexpr int PER_WAVE = BLOCKS / NW;
     static_assert(PER_WAVE * NW == BLOCKS, "blocks must split evenly over the wavefronts");
     static constexpr int CPR = ROWS / 8;  // chunks per k-row, row-contiguous layout
     static constexpr int KPB = 64 / CPR;  // k-rows per block, row-contiguous layout
-    static_assert(KCONTIG || (CPR >= 8 && CPR <= 64 && 64 % CPR == 0), "row-contiguous tile: 64..512 rows");
+    static_assert(KCONTIG || (sizeof(T) == 2 && CPR >= 8 && CPR <= 64 && 64 % CPR == 0),
+                  "row-contiguous tile: 2-byte elements, 64..512 rows");
 
     __device__ static __forceinline__ int tswz(int k) { return 4 * (k & 1); }
 
     // Wavefront w fetches its blocks of the stage starting at k0 into `lds`.
-    __device__ static __forceinline__ void issue(const __bf16* src, int64_t ld, int r0, int k0, char* lds, int w,
+    __device__ static __forceinline__ void issue(const T* src, int64_t ld, int r0, int k0, char* lds, int w,
                                                  int lane) {
 #pragma unroll
         for (int i = 0; i < PER_WAVE; ++i) {
             const int b = i * NW + w;
-            const __bf16* g;
+            const T* g;
             if constexpr (KCONTIG) {
                 const int row = 8 * b + (lane >> 3);
-                g = src + (int64_t)(r0 + row) * ld + k0 + 8 * ((lane & 7) ^ (row & 7));
+                g = src + (int64_t)(r0 + row) * ld + k0 + EPC * ((lane & 7) ^ (row & 7));
             } else {
                 const int k = b * KPB + lane / CPR;
-                g = src + (int64_t)(k0 + k) * ld + r0 + 8 * ((lane % CPR) ^ tswz(k));
+                g = src + (int64_t)(k0 + k) * ld + r0 + EPC * ((lane % CPR) ^ tswz(k));
             }
             __builtin_amdgcn_global_load_lds((glb_void_t*)g, (lds_void_t*)(lds + b * 1024), 16, 0, 0);
         }
     }
 
-    // MFMA operand (8 k values of row rbase + lane%32 at k = kk + 8*(lane/32))
+    // 16-byte chunk c (k bytes 16c .. 16c+15) of row `row` (k-contiguous layout)
+    __device__ static __forceinline__ u32x4 chunk(const char* lds, int row, int c) {
+        return *(const u32x4*)(lds + row * 128 + 16 * (c ^ (row & 7)));
+    }
+
+    // BF16 MFMA operand (8 k values of row rbase + lane%32 at k = kk + 8*(lane/32))
     __device__ static __forceinline__ bf16x8 frag(const char* lds, int rbase, int kk, int lane) {
         if constexpr (KCONTIG) {
             const int row = rbase + (lane & 31);

@@ -15,6 +15,11 @@
 // its B block, 32 bytes each at k = kk + 32*(l/32) .. +31; A and B use the
 // same k assignment, so the 64-deep sum pairs bytes of equal k.  K tiles are
 // double-buffered through LDS, rows k-contiguous at a 144-byte stride.
+// Interior tiles (whole 128-deep K tiles, 16-byte aligned rows) stage both
+// operands straight from HBM into LDS (global_load_lds, rows XOR-swizzled
+// as in bf16_dma_tile.hpp) under a software-pipelined K loop; edge tiles
+// stage through registers.
+#include "bf16_dma_tile.hpp"
 #include "mfma_common.hpp"
 
 namespace gat {
@@ -31,17 +36,149 @@ struct Fp8Shape {
     static constexpr int NT = WM * WN * 64;
     static constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     static constexpr size_t SMEM = 2 * (size_t)(BM + BN) * F8_LDK;
-    static_assert(TM * 32 <= 128 && TN * 32 <= 128, "a wave must stay inside one scale block");
+    static_assert(TM * 32 <= 128 && TN * 32 <= 128 && 128 % (TM * 32) == 0 && 128 % (TN * 32) == 0,
+                  "a wave must stay inside one scale block");
     static_assert((BM * (F8_BK / 16)) % NT == 0 && (BN * (F8_BK / 16)) % NT == 0, "16-byte chunks per thread");
 };
 
+// Waves per SIMD the LDS footprint allows, capped by 512 registers over the
+// accumulators (16 per block) and a working set; the register allocator is
+// asked for at least that many.
 template <int WM, int WN, int TM, int TN>
-__global__ __launch_bounds__(WM* WN * 64) void gemm_fp8_kernel(const u8* __restrict__ A, const u8* __restrict__ B,
-                                                               const float* __restrict__ sA,
-                                                               const float* __restrict__ sB, float* __restrict__ C,
-                                                               int M, int N, int K, int64_t ldc, float alpha,
-                                                               float beta, int tiles_m, int tiles_n, int group_m,
-                                                               int vec_ok) {
+constexpr int fp8_waves_per_simd() {
+    constexpr int by_lds = (int)(163840 / Fp8Shape<WM, WN, TM, TN>::SMEM) * WM * WN / 4;
+    constexpr int by_regs = 512 / (TM * TN * 16 + 128);
+    constexpr int w = by_lds < by_regs ? by_lds : by_regs;
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+// The BM x BN tile at (m0, n0), wholly inside M x N, K a multiple of 128:
+// direct-to-LDS stages, fragments of the next 64-deep step read under the
+// MFMAs of the current one, the stage wait and barrier placed before the
+// last step (see bf16_tile_dma).  arb / brb: this wave's scale-block rows.
+template <int WM, int WN, int TM, int TN>
+__device__ __forceinline__ void fp8_tile_dma(const u8* __restrict__ A, const u8* __restrict__ B,
+                                             const float* __restrict__ sA, const float* __restrict__ sB,
+                                             float* __restrict__ C, int K, int64_t ldc, float alpha, float beta,
+                                             int m0, int n0, int arb, int brb, u8* smem) {
+    constexpr int NW = WM * WN, BM = WM * TM * 32, BN = WN * TN * 32;
+    typedef DmaOperand<BM, true, NW, u8> OpA;
+    typedef DmaOperand<BN, true, NW, u8> OpB;
+    constexpr int STAGE = OpA::BYTES + OpB::BYTES;
+    char* const sb = (char*)smem;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = w / WN, wn = w % WN;
+    const int KT = K / F8_BK;
+
+    f16v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    i32x8 a0[TM], b0[TN], a1[TM], b1[TN];
+    // lane l: row l%32 of each block, 32 bytes at k = kk + 32*(l/32)
+    auto frags = [&](const char* stage, int kk, i32x8(&a)[TM], i32x8(&b)[TN]) {
+        const int c = kk / 16 + 2 * (lane >> 5);
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+            const int row = wm * TM * 32 + i * 32 + (lane & 31);
+            const u32x4 lo = OpA::chunk(stage, row, c), hi = OpA::chunk(stage, row, c + 1);
+            a[i] = i32x8{(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3], (int)hi[0], (int)hi[1], (int)hi[2],
+                         (int)hi[3]};
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int row = wn * TN * 32 + j * 32 + (lane & 31);
+            const u32x4 lo = OpB::chunk(stage + OpA::BYTES, row, c), hi = OpB::chunk(stage + OpA::BYTES, row, c + 1);
+            b[j] = i32x8{(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3], (int)hi[0], (int)hi[1], (int)hi[2],
+                         (int)hi[3]};
+        }
+    };
+    auto mma = [&](const i32x8(&a)[TM], const i32x8(&b)[TN], int sa, int sbb) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[i], b[j], acc[i][j], 0, 0, 0, sa, 0,
+                                                                           sbb);
+    };
+    auto interleave = [&]() {
+#pragma unroll
+        for (int i = 0; i < TM * TN; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // LDS reads
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load (to LDS)
+        }
+    };
+    const float* sAr = sA + (int64_t)arb * KT;
+    const float* sBr = sB + (int64_t)brb * KT;
+
+    OpA::issue(A, K, m0, 0, sb, w, lane);
+    OpB::issue(B, K, n0, 0, sb + OpA::BYTES, w, lane);
+    int sa_c = e8m0_of(sAr[0]), sb_c = e8m0_of(sBr[0]);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    frags(sb, 0, a0, b0);
+
+    // Two fragment sets in flight need 2*(TM+TN)*8 registers beside the
+    // accumulators; where that overflows the 256 of a two-wave SIMD, each
+    // step reads its own fragments and the SIMD's other wave covers the LDS
+    // latency.
+    constexpr bool kPipelined = TM * TN * 16 + 2 * (TM + TN) * 8 <= 176;
+    for (int t = 0; t < KT; ++t) {
+        const char* cur = sb + (t & 1) * STAGE;
+        char* nxt = sb + ((t + 1) & 1) * STAGE;
+        const int tn = t + 1 < KT ? t + 1 : t;  // the last iteration fetches its own stage again
+        OpA::issue(A, K, m0, tn * F8_BK, nxt, w, lane);
+        OpB::issue(B, K, n0, tn * F8_BK, nxt + OpA::BYTES, w, lane);
+        const int sa_n = e8m0_of(sAr[tn]), sb_n = e8m0_of(sBr[tn]);
+        if constexpr (kPipelined) {
+            frags(cur, 64, a1, b1);
+            mma(a0, b0, sa_c, sb_c);
+            interleave();
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
+            __syncthreads();
+            frags(nxt, 0, a0, b0);
+            mma(a1, b1, sa_c, sb_c);
+            interleave();
+        } else {
+            __builtin_amdgcn_iglp_opt(0);
+            mma(a0, b0, sa_c, sb_c);
+            frags(cur, 64, a0, b0);
+            mma(a0, b0, sa_c, sb_c);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
+            __syncthreads();
+            frags(nxt, 0, a0, b0);
+        }
+        sa_c = sa_n;
+        sb_c = sb_n;
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * (TM * 32) + i * 32 + mfma32_row(lane, r);
+                const int col = n0 + wn * (TN * 32) + j * 32 + mfma32_col(lane);
+                float* p = C + (int64_t)row * ldc + col;
+                float v = alpha * acc[i][j][r];
+                if (beta != 0.0f) v += beta * gload<float>(p);
+                gstore(p, v);
+            }
+}
+
+template <int WM, int WN, int TM, int TN>
+__global__ __launch_bounds__(WM* WN * 64)
+__attribute__((amdgpu_waves_per_eu(fp8_waves_per_simd<WM, WN, TM, TN>()))) void gemm_fp8_kernel(
+    const u8* __restrict__ A, const u8* __restrict__ B, const float* __restrict__ sA, const float* __restrict__ sB,
+    float* __restrict__ C, int M, int N, int K, int64_t ldc, float alpha, float beta, int tiles_m, int tiles_n,
+    int group_m, int vec_ok, int skip_full) {
     typedef Fp8Shape<WM, WN, TM, TN> S;
     constexpr int NT = S::NT, BM = S::BM, BN = S::BN;
     constexpr int QA = BM * (F8_BK / 16) / NT, QB = BN * (F8_BK / 16) / NT;  // chunks per thread
@@ -57,6 +194,7 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_fp8_kernel(const u8* __restr
     // edge are computed but never stored)
     const int mbl = (M + 127) / 128, nbl = (N + 127) / 128;
     const int arb = min((m0 + wm * TM * 32) / 128, mbl - 1), brb = min((n0 + wn * TN * 32) / 128, nbl - 1);
+    if (skip_full && m0 + BM <= M && n0 + BN <= N) return;  // interior tile: the direct-to-LDS kernel's
 
     f16v acc[TM][TN];
 #pragma unroll
@@ -76,7 +214,7 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_fp8_kernel(const u8* __restr
             for (int e = 0; e < 16 && gk + e < K; ++e) v[e >> 2] |= (unsigned)p[e] << (8 * (e & 3));
         return v;
     };
-    auto gload = [&](int t) {
+    auto fetch = [&](int t) {
         const int k0 = t * F8_BK;
         const bool full = full_mn && k0 + F8_BK <= K;
 #pragma unroll
@@ -103,7 +241,7 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_fp8_kernel(const u8* __restr
         }
     };
 
-    gload(0);
+    fetch(0);
     sstore(lds8);
     __syncthreads();
     const int arow = wm * (TM * 32) + (lane & 31);
@@ -111,7 +249,7 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_fp8_kernel(const u8* __restr
     const int kh = 32 * (lane >> 5);
     for (int t = 0; t < kblocks; ++t) {
         const u8* buf = lds8 + (t & 1) * ((BM + BN) * F8_LDK);
-        if (t + 1 < kblocks) gload(t + 1);
+        if (t + 1 < kblocks) fetch(t + 1);
         const int sa = e8m0_of(sA[(int64_t)arb * kblocks + t]);
         const int sb = e8m0_of(sB[(int64_t)brb * kblocks + t]);
 #pragma unroll
@@ -157,11 +295,32 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_fp8_kernel(const u8* __restr
             }
 }
 
+// Interior tiles only (grid = (M/BM) x (N/BN) whole tiles), K a multiple of
+// 128, 16-byte aligned rows: direct-to-LDS staging.  A kernel of its own so
+// that its registers are allocated for this path alone.
+template <int WM, int WN, int TM, int TN>
+__global__ __launch_bounds__(WM* WN * 64)
+__attribute__((amdgpu_waves_per_eu(fp8_waves_per_simd<WM, WN, TM, TN>()))) void gemm_fp8_dma_kernel(
+    const u8* __restrict__ A, const u8* __restrict__ B, const float* __restrict__ sA, const float* __restrict__ sB,
+    float* __restrict__ C, int M, int N, int K, int64_t ldc, float alpha, float beta, int tiles_m, int tiles_n,
+    int group_m) {
+    typedef Fp8Shape<WM, WN, TM, TN> S;
+    extern __shared__ __attribute__((aligned(16))) u8 lds8[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave / WN, wn = wave % WN;
+    int tmi, tni;
+    mfma_tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, group_m, tmi, tni);
+    const int m0 = tmi * S::BM, n0 = tni * S::BN;
+    const int arb = (m0 + wm * TM * 32) / 128, brb = (n0 + wn * TN * 32) / 128;
+    fp8_tile_dma<WM, WN, TM, TN>(A, B, sA, sB, C, K, ldc, alpha, beta, m0, n0, arb, brb, lds8);
+}
+
 // Instantiated configurations (WM, WN, TM, TN); the K depth is always 128.
 #define GAT_FP8_CFGS(X) \
     X(2, 2, 2, 2)       \
     X(2, 4, 2, 2)       \
-    X(4, 2, 2, 2)
+    X(4, 2, 2, 2)       \
+    X(2, 4, 4, 2)
 
 static const MfmaCfg kFp8Cfgs[] = {
 #define X(wm, wn, tm, tn) {wm, wn, tm, tn, F8_BK, 32},
@@ -185,11 +344,27 @@ static int launch_fp8_one(const GemmArgs& a, hipStream_t st) {
     const int tiles_m = (int)((a.M + S::BM - 1) / S::BM), tiles_n = (int)((a.N + S::BN - 1) / S::BN);
     const int vec_ok = ((uintptr_t)a.A % 16 == 0) && ((uintptr_t)a.B % 16 == 0) && (a.K % 16 == 0);
     const int group_m = tiles_m < 8 ? tiles_m : 8;
+    // whole tiles through the direct-to-LDS kernel, the rest (or everything,
+    // when K is not a multiple of 128 or rows are unaligned) through the
+    // register-staged one
+    const bool dma = vec_ok && a.K >= F8_BK && a.K % F8_BK == 0;
+    const int full_m = (int)(a.M / S::BM), full_n = (int)(a.N / S::BN);
+    if (dma && full_m > 0 && full_n > 0) {
+        const void* fd = (const void*)gemm_fp8_dma_kernel<WM, WN, TM, TN>;
+        if (S::SMEM > 65536) GAT_HIP_TRY(allow_dynamic_lds(fd, S::SMEM));
+        const int gm = full_m < 8 ? full_m : 8;
+        hipLaunchKernelGGL((gemm_fp8_dma_kernel<WM, WN, TM, TN>), dim3((unsigned)(full_m * full_n)), dim3(S::NT),
+                           S::SMEM, st, (const u8*)a.A, (const u8*)a.B, a.scaleA, a.scaleB, (float*)a.C, (int)a.M,
+                           (int)a.N, (int)a.K, a.ldc, (float)a.alpha, (float)a.beta, full_m, full_n, gm);
+        GAT_HIP_TRY(hipGetLastError());
+        if (full_m == tiles_m && full_n == tiles_n) return ST_OK;
+    }
+    const int skip_full = dma && full_m > 0 && full_n > 0;
     const void* f = (const void*)gemm_fp8_kernel<WM, WN, TM, TN>;
     if (S::SMEM > 65536) GAT_HIP_TRY(allow_dynamic_lds(f, S::SMEM));
     hipLaunchKernelGGL((gemm_fp8_kernel<WM, WN, TM, TN>), dim3((unsigned)(tiles_m * tiles_n)), dim3(S::NT), S::SMEM,
                        st, (const u8*)a.A, (const u8*)a.B, a.scaleA, a.scaleB, (float*)a.C, (int)a.M, (int)a.N,
-                       (int)a.K, a.ldc, (float)a.alpha, (float)a.beta, tiles_m, tiles_n, group_m, vec_ok);
+                       (int)a.K, a.ldc, (float)a.alpha, (float)a.beta, tiles_m, tiles_n, group_m, vec_ok, skip_full);
     GAT_HIP_TRY(hipGetLastError());
     return ST_OK;
 }

@@ -224,6 +224,52 @@ def test_fp8_every_cfg(ctx, cfg, M, N, K):
     assert cw <= TOL, (cfg, cw)
 
 
+class _Ptr:
+    """A raw device address standing in for a DeviceArray argument."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+@pytest.mark.parametrize("cfg", Context.tile_cfgs(_capi.GAT_FP8), ids=str)
+def test_fp8_direct_lds_path(ctx, cfg):
+    """Resident FP8 operands, K a multiple of 128, ragged M and N: whole tiles
+    run in the direct-to-LDS kernel, edge tiles in the register-staged one.
+    The same bytes placed one byte past 16-byte alignment send every tile
+    through the register-staged kernel; both results agree bit for bit and
+    with the FP64 product of the dequantised operands."""
+    import ctypes
+    M, N, K = 300, 520, 384
+    _say("fp8 direct-LDS", cfg, M, N, K)
+    A, B, C0 = _operands(M, N, K, 0, 0, 36)
+    dA, dB = ctx.to_device(A), ctx.to_device(B)
+    kb = K // 128
+    A8, B8 = ctx.empty(M * K, np.uint8), ctx.empty(N * K, np.uint8)
+    sA, sB = ctx.empty(((M + 127) // 128) * kb, np.float32), ctx.empty(((N + 127) // 128) * kb, np.float32)
+    ctx.quantize_fp8(dA, M, K, K, False, A8, sA)
+    ctx.quantize_fp8(dB, N, K, N, True, B8, sB)
+    A8s, B8s = ctx.empty(M * K + 16, np.uint8), ctx.empty(N * K + 16, np.uint8)
+    _capi.check(ctx.lib.gat_memcpy_d2d(ctx.h, A8s.ptr + 1, A8.ptr, M * K), "d2d")
+    _capi.check(ctx.lib.gat_memcpy_d2d(ctx.h, B8s.ptr + 1, B8.ptr, N * K), "d2d")
+    d1, d2 = ctx.to_device(C0), ctx.to_device(C0)
+    # resident FP8 GEMMs use the tuned tile of their (FP8, NT) size bucket
+    _capi.check(ctx.lib.gat_tuned_set(ctx.h, _capi.GAT_FP8, 0, 1, M, N, K, ctypes.byref(_capi.TileCfg(*cfg))),
+                "tuned_set")
+    try:
+        ctx.gemm_fp8(A8, sA, B8, sB, d1, M, N, K, alpha=1.5, beta=-0.5)
+        ctx.gemm_fp8(_Ptr(A8s.ptr + 1), sA, _Ptr(B8s.ptr + 1), sB, d2, M, N, K, alpha=1.5, beta=-0.5)
+        got, ref = d1.download(), d2.download()
+    finally:
+        ctx.tuned_clear()
+    Ad = dequant_fp8(A8.download(), sA.download(), M, K)
+    Bd = dequant_fp8(B8.download(), sB.download(), N, K)
+    C64 = 1.5 * Ad @ Bd.T - 0.5 * C0.reshape(M, N).astype(np.float64)
+    MAG = 1.5 * np.abs(Ad) @ np.abs(Bd).T + 0.5 * np.abs(C0.reshape(M, N))
+    cw, _, _ = ora.error_metrics(got, C64, MAG)
+    assert cw <= TOL, (cfg, cw)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), cfg
+
+
 @pytest.mark.parametrize("dt", [_capi.GAT_BF16X3, _capi.GAT_FP8, _capi.GAT_BF16])
 def test_mixed_test_pattern(ctx, dt):
     M = N = K = 1024
