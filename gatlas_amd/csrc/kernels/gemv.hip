@@ -29,15 +29,16 @@ __global__ __launch_bounds__(512) void gemv_rows_kernel(int64_t M, int64_t N, T 
     gemv_rows_wave<T, V, ROWS, UNROLL>(M, N, alpha, A, lda, x, beta, y, vec_ok, r0, threadIdx.x & 63);
 }
 
-// Transposed: pass 1 partial sums.  Block = 256 work-items covering
-// 256*W consecutive outputs; blockIdx.y = slice.
-template <typename T, typename V>
-__global__ __launch_bounds__(256) void gemv_cols_partial_kernel(int64_t M, int64_t N, const T* __restrict__ A,
+// Transposed: pass 1 partial sums.  A work-group of wg work-items covers
+// wg*W consecutive outputs; blockIdx.y = slice of the j range.  Each
+// work-item keeps JU rows of A in flight per step.
+template <typename T, typename V, int JU>
+__global__ __launch_bounds__(512) void gemv_cols_partial_kernel(int64_t M, int64_t N, const T* __restrict__ A,
                                                                  int64_t lda, const T* __restrict__ x,
                                                                  T* __restrict__ part, int64_t jslice,
                                                                  int vec_ok) {
     constexpr int W = sizeof(V) / sizeof(T);
-    const int64_t i0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * W;
+    const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * W;
     const int64_t j0 = (int64_t)blockIdx.y * jslice;
     int64_t j1 = j0 + jslice;
     if (j1 > N) j1 = N;
@@ -46,18 +47,15 @@ __global__ __launch_bounds__(256) void gemv_cols_partial_kernel(int64_t M, int64
     if (vec_ok && i0 + W <= M) {
         V acc = {};
         int64_t j = j0;
-        for (; j + 4 <= j1; j += 4) {
-            V a0 = __builtin_nontemporal_load((const V*)(A + (j + 0) * lda + i0));
-            V a1 = __builtin_nontemporal_load((const V*)(A + (j + 1) * lda + i0));
-            V a2 = __builtin_nontemporal_load((const V*)(A + (j + 2) * lda + i0));
-            V a3 = __builtin_nontemporal_load((const V*)(A + (j + 3) * lda + i0));
-            acc += a0 * x[j + 0];
-            acc += a1 * x[j + 1];
-            acc += a2 * x[j + 2];
-            acc += a3 * x[j + 3];
+        for (; j + JU <= j1; j += JU) {
+            V a[JU];
+#pragma unroll
+            for (int u = 0; u < JU; ++u) a[u] = gload_nt<V>(A + (j + u) * lda + i0);
+#pragma unroll
+            for (int u = 0; u < JU; ++u) acc += a[u] * gload<T>(x + j + u);
         }
-        for (; j < j1; ++j) acc += *(const V*)(A + j * lda + i0) * x[j];
-        *(V*)(out + i0) = acc;
+        for (; j < j1; ++j) acc += gload<V>(A + j * lda + i0) * gload<T>(x + j);
+        gstore(out + i0, acc);
     } else {
 #pragma unroll
         for (int e = 0; e < W; ++e) {
@@ -81,20 +79,25 @@ __global__ __launch_bounds__(256) void gemv_cols_reduce_kernel(int64_t M, int S,
     y[i] = v;
 }
 
-static int gemv_cols_slices(size_t M, size_t N, int W) {
-    const size_t iblocks = (M + 256 * W - 1) / (256 * W);
-    size_t S = (2048 + iblocks - 1) / iblocks;
-    const size_t maxS = (N + 63) / 64;  // keep >= 64 rows of A per slice
+// Slices of the j range for `target` work-groups in all, at least 64 rows
+// of A each, at most 1024.
+static size_t gemv_cols_max_slices(size_t N) {
+    const size_t s = (N + 63) / 64;
+    return s < 1 ? 1 : (s > 1024 ? 1024 : s);
+}
+static int gemv_cols_slices(size_t M, size_t N, int W, int wg, size_t target) {
+    const size_t iblocks = (M + (size_t)wg * W - 1) / ((size_t)wg * W);
+    size_t S = (target + iblocks - 1) / iblocks;
+    const size_t maxS = gemv_cols_max_slices(N);
     if (S > maxS) S = maxS;
     if (S < 1) S = 1;
-    if (S > 1024) S = 1024;
     return (int)S;
 }
 
+// The partial-sum workspace of the transposed form, for any launch shape.
 size_t gemv_workspace_bytes(int dtype, int transA, size_t M, size_t N) {
     if (!transA) return 0;
-    const int W = dtype == DT_F64 ? 2 : 4;
-    return (size_t)gemv_cols_slices(M, N, W) * M * (dtype == DT_F64 ? 8 : 4);
+    return gemv_cols_max_slices(N) * M * (dtype == DT_F64 ? 8 : 4);
 }
 
 template <typename T, typename V, int ROWS, int UNROLL>
@@ -136,14 +139,22 @@ static int launch_native_t(int transA, const GemvCfg& cfg, size_t M, size_t N, T
         set_error("gemv transposed: workspace required");
         return ST_INVALID;
     }
-    const int S = gemv_cols_slices(M, N, W);
+    // launch shape of the transposed form: cfg.wg work-items per group,
+    // 2*cfg.rows rows of A in flight per work-item, 1024*cfg.unroll
+    // work-groups in all (j range split into slices to reach them)
+    const int S = gemv_cols_slices(M, N, W, cfg.wg, (size_t)1024 * cfg.unroll);
     const int64_t jslice = (int64_t)((N + S - 1) / S);
     // partial rows start at slice*M: vector stores need M % W == 0
     const int vec_ok = ((uintptr_t)A % 16 == 0) && (lda % W == 0) && (M % W == 0) &&
                        ((uintptr_t)work % 16 == 0);
-    const size_t iblocks = (M + 256 * W - 1) / (256 * W);
-    hipLaunchKernelGGL((gemv_cols_partial_kernel<T, V>), dim3((unsigned)iblocks, (unsigned)S), dim3(256), 0, st,
-                       (int64_t)M, (int64_t)N, A, (int64_t)lda, x, (T*)work, jslice, vec_ok);
+    const size_t iblocks = (M + (size_t)cfg.wg * W - 1) / ((size_t)cfg.wg * W);
+    const dim3 grid((unsigned)iblocks, (unsigned)S);
+#define GAT_COLS(JU)                                                                                       \
+    if (cfg.rows * 2 == JU)                                                                                \
+        hipLaunchKernelGGL((gemv_cols_partial_kernel<T, V, JU>), grid, dim3((unsigned)cfg.wg), 0, st,       \
+                           (int64_t)M, (int64_t)N, A, (int64_t)lda, x, (T*)work, jslice, vec_ok);
+    GAT_COLS(2) GAT_COLS(4) GAT_COLS(8) GAT_COLS(16)
+#undef GAT_COLS
     GAT_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL((gemv_cols_reduce_kernel<T>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st,
                        (int64_t)M, S, (const T*)work, alpha, beta, y);
@@ -154,10 +165,14 @@ static int launch_native_t(int transA, const GemvCfg& cfg, size_t M, size_t N, T
 int launch_gemv_native(int dtype, int transA, size_t M, size_t N, double alpha, const void* A, size_t lda,
                        const void* x, double beta, void* y, void* work, hipStream_t st, const GemvCfg* cfg) {
     if (M == 0) return ST_OK;
-    // default launch shape: the fastest of the 16384^2 sweep (128 work-items,
-    // 2 rows per wave, one load per row in flight); the grouped kernel uses
-    // the same rows/unroll so results match bit for bit
-    const GemvCfg c = cfg ? *cfg : GemvCfg{128, GEMV_GROUPED_ROWS, GEMV_GROUPED_UNROLL};
+    // default launch shapes: the fastest of the 16384^2 sweep.  Rows: 128
+    // work-items, 2 rows per wave, one load per row in flight (the grouped
+    // kernel uses the same rows/unroll so results match bit for bit).
+    // Transposed: 64 work-items (256 outputs) per group, 8 rows of A in
+    // flight per work-item, 1024 work-groups.
+    const GemvCfg c = cfg ? *cfg
+                          : transA ? GemvCfg{64, 4, 1}
+                                   : GemvCfg{128, GEMV_GROUPED_ROWS, GEMV_GROUPED_UNROLL};
     if (dtype == DT_F32)
         return launch_native_t<float, f4v>(transA, c, M, N, (float)alpha, (const float*)A, lda, (const float*)x,
                                            (float)beta, (float*)y, work, st);

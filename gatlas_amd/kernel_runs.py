@@ -128,22 +128,26 @@ def sweep_tiles(ctx, n, forms, reps):
 def sweep_bandwidth(ctx, reps=20):
     """Launch shapes of the bandwidth kernels through the parameter interface
     (Bench::run, `reps` chained launches timed on the host): GEMV y = A*x,
-    16384^2 (work-group, rows per wave, unroll digit) and SAXPY z = 1.5x + y,
-    n = 2^28 (work-group, vectors per work-item, non-temporal digit)."""
+    16384^2 (work-group, rows, unroll digit; rows = rows per wave, or rows
+    of A in flight / 2 in the transposed form, whose unroll digit sets the
+    work-group count) and SAXPY z = 1.5x + y, n = 2^28 (work-group, vectors
+    per work-item, non-temporal digit)."""
     from .interface import MATVEC, NATIVE, SAXPY, Kernel
     m, n = 16384, 1 << 28
     k = Kernel(MATVEC, _capi.GAT_F32, 4, NATIVE, ctx=ctx)
     print("| kernel | params | us | GB/s |\n|---|---|---|---|", flush=True)
-    for wg in (64, 128, 256, 512):
-        for rows in (1, 2, 4, 8):
-            for xp in (0, 1, 2):
-                p = [1, 0, m, m, 0, wg, rows, 4, xp]
-                k.setParams(p)
-                if not k.validParams():
-                    continue
-                k.setArgs(True)
-                us = k.run(reps) / reps
-                print("| gemv | %s | %.1f | %.0f |" % (p, us, 4.0 * (m * m + 2 * m) / max(us, 1e-3) / 1e3), flush=True)
+    for tr in (0, 1):
+        for wg in (64, 128, 256, 512):
+            for rows in (1, 2, 4, 8):
+                for xp in (0, 1, 2):
+                    p = [1, 0, m, m, tr, wg, rows, 4, xp]
+                    k.setParams(p)
+                    if not k.validParams():
+                        continue
+                    k.setArgs(True)
+                    us = k.run(reps) / reps
+                    print("| gemv%s | %s | %.1f | %.0f |" % ("T" if tr else "", p, us,
+                                                           4.0 * (m * m + 2 * m) / max(us, 1e-3) / 1e3), flush=True)
     s = Kernel(SAXPY, _capi.GAT_F32, 4, NATIVE, ctx=ctx)
     for wg in (64, 128, 256, 512, 1024):
         for un in (1, 2, 4, 8):

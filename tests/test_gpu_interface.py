@@ -55,6 +55,25 @@ def test_kernel_interface_trial(ctx, ci, mode):
         assert err == 0.0  # bit-exact against the reference's host check
 
 
+@pytest.mark.parametrize("wg", [64, 128, 256, 512])
+@pytest.mark.parametrize("rows", [1, 2, 4, 8])
+def test_native_matvec_transposed_launch_shapes(ctx, wg, rows):
+    # every launch shape of the transposed native GEMV (rows of A in flight,
+    # work-group count digit), ragged M and N, checked by the interface's
+    # host check in paranoid mode
+    for xp in (0, 1, 2):
+        params = [1, 1, 1000, 1028, 1, wg, rows, 4, xp]
+        print("[case]", params, flush=True)
+        k = Kernel(MATVEC, _capi.GAT_F32, 4, NATIVE, ctx=ctx)
+        k.setParams(params)
+        assert k.validParams(), params
+        assert k.build()
+        k.paranoidCheck()
+        us = k.run(1)
+        ok, err = k.checkOutput()
+        assert us > 0 and ok, (params, err)
+
+
 def _run(cmd, timeout=240):
     print("[cmd]", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
