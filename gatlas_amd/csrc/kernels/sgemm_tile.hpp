@@ -175,8 +175,14 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
         const float* As = smem + (t & 1) * (BM + BN) * LDK;
         const float* Bs = As + BM * LDK;
         if (t + 1 < KT) fetch(t + 1);
+        // The next stage goes to LDS halfway through this one's steps, not
+        // after the last: its stores, and the barrier after them, then
+        // overlap the MFMAs of the second half still in the pipeline.
+        constexpr int NSTEP = BK / Op::KGROUP;
+        constexpr int STORE_AT = NSTEP / 2 - 1 < 0 ? 0 : NSTEP / 2 - 1;
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += Op::KGROUP) {
+        for (int st = 0; st < NSTEP; ++st) {
+            const int kk = st * Op::KGROUP;
             f4v a[TM], b[TN];
 #pragma unroll
             for (int i = 0; i < TM; ++i) a[i] = *(const f4v*)(As + (arow + i * BLK) * LDK + kk + kq);
@@ -188,11 +194,11 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
                     for (int j = 0; j < TN; ++j) acc[i][j] = Op::mma(a[i][s], b[j][s], acc[i][j]);
-        }
-        if (t + 1 < KT) {
-            float* nA = smem + ((t + 1) & 1) * (BM + BN) * LDK;
-            IoA::store(ra, nA, tid);
-            IoB::store(rb, nA + BM * LDK, tid);
+            if (st == STORE_AT && t + 1 < KT) {
+                float* nA = smem + ((t + 1) & 1) * (BM + BN) * LDK;
+                IoA::store(ra, nA, tid);
+                IoB::store(rb, nA + BM * LDK, tid);
+            }
         }
         __syncthreads();
     }
