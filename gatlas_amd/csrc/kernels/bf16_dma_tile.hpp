@@ -64,6 +64,29 @@ struct DmaOperand {
         }
     }
 
+    // The stage of k values [k0, k0 + klen) (klen < 64 a multiple of EPC;
+    // the rest of the 64 zero) written into `lds` in this operand's layout
+    // through registers, by all nt work-items: the first stage of a K range
+    // that is not a multiple of 64.
+    __device__ static __forceinline__ void stage_head(const T* src, int64_t ld, int r0, int k0, int klen, char* lds,
+                                                      int tid, int nt) {
+        if constexpr (KCONTIG) {
+            for (int q = tid; q < ROWS * 8; q += nt) {
+                const int row = q >> 3, c = q & 7;
+                u32x4 v = {0u, 0u, 0u, 0u};
+                if (EPC * c < klen) v = gload<u32x4>(src + (int64_t)(r0 + row) * ld + k0 + EPC * c);
+                *(u32x4*)(lds + row * 128 + 16 * (c ^ (row & 7))) = v;
+            }
+        } else {
+            for (int q = tid; q < 64 * CPR; q += nt) {
+                const int k = q / CPR, c = q % CPR;
+                u32x4 v = {0u, 0u, 0u, 0u};
+                if (k < klen) v = gload<u32x4>(src + (int64_t)(k0 + k) * ld + r0 + EPC * c);
+                *(u32x4*)(lds + k * (ROWS * 2) + 16 * (c ^ tswz(k))) = v;
+            }
+        }
+    }
+
     // 16-byte chunk c (k bytes 16c .. 16c+15) of row `row` (k-contiguous layout)
     __device__ static __forceinline__ u32x4 chunk(const char* lds, int row, int c) {
         return *(const u32x4*)(lds + row * 128 + 16 * (c ^ (row & 7)));
@@ -96,8 +119,10 @@ struct Bf16DmaShape {
 };
 
 // Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C for
-// a tile wholly inside M x N with K a multiple of 64 and 16-byte aligned
-// rows; all WM*WN*64 threads of the group call it; ends after a barrier.
+// a tile wholly inside M x N, K >= 64 a multiple of 8, 16-byte aligned rows;
+// all WM*WN*64 threads of the group call it; ends after a barrier.  When K
+// is not a multiple of 64 the first stage holds the K % 64 leading k values
+// (staged through registers, zero-padded) and every later stage is whole.
 template <int WM, int WN, int TM, int TN, bool TA, bool TB>
 __device__ __forceinline__ void bf16_tile_dma(const __bf16* __restrict__ A, const __bf16* __restrict__ B,
                                               float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb,
@@ -120,7 +145,10 @@ __device__ __forceinline__ void bf16_tile_dma(const __bf16* __restrict__ A, cons
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    const int KT = K / 64;
+    const int kh = K % 64;  // k values of a short first stage (0: none)
+    const int KT = K / 64 + (kh != 0);
+    // first k of stage s
+    auto kbase = [&](int st) { return st == 0 ? 0 : st * 64 - (kh ? 64 - kh : 0); };
     bf16x8 a0[TM], b0[TN], a1[TM], b1[TN];
     auto frags = [&](const char* stage, int kk, bf16x8(&a)[TM], bf16x8(&b)[TN]) {
 #pragma unroll
@@ -145,8 +173,13 @@ __device__ __forceinline__ void bf16_tile_dma(const __bf16* __restrict__ A, cons
         }
     };
 
-    OpA::issue(A, lda, m0, 0, sb, w, lane);
-    OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
+    if (kh) {
+        OpA::stage_head(A, lda, m0, 0, kh, sb, tid, NW * 64);
+        OpB::stage_head(B, ldb, n0, 0, kh, sb + OpA::BYTES, tid, NW * 64);
+    } else {
+        OpA::issue(A, lda, m0, 0, sb, w, lane);
+        OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
+    }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     frags(sb, 0, a0, b0);
@@ -161,7 +194,7 @@ __device__ __forceinline__ void bf16_tile_dma(const __bf16* __restrict__ A, cons
     for (int t = 0; t < KT; ++t) {
         const char* cur = sb + (t & 1) * STAGE;
         char* nxt = sb + ((t + 1) & 1) * STAGE;
-        const int kn = (t + 1 < KT ? t + 1 : t) * 64;
+        const int kn = kbase(t + 1 < KT ? t + 1 : t);  // K >= 64: the last stage is whole
         OpA::issue(A, lda, m0, kn, nxt, w, lane);
         OpB::issue(B, ldb, n0, kn, nxt + OpA::BYTES, w, lane);
         frags(cur, 16, a1, b1);

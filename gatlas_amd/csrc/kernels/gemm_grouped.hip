@@ -18,6 +18,10 @@
 //
 // Tile shapes: 0 = 128 x 128 (2 x 2 waves of 2 x 2 32x32 blocks),
 //              1 = 64 x 64 (2 x 2 waves of one 32x32 block).
+// BF16 interior tiles of problems with K >= 64 a multiple of 8 and 16-byte
+// aligned rows stage straight into LDS (bf16_dma_tile.hpp); the others, and
+// every FP32 tile, stage through registers.
+#include "bf16_dma_tile.hpp"
 #include "bf16_tile.hpp"
 #include "grouped.hpp"
 #include "sgemm_tile.hpp"
@@ -39,12 +43,24 @@ template <bool TA, bool TB>
 __device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0,
                                                 __bf16* smem) {
     const bool vec = (p.flags & GP_VEC) != 0;
-    if (shape == 0)
-        bf16_tile<2, 2, 2, 2, 32, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
-                                                     p.beta, m0, n0, vec, smem);
-    else
-        bf16_tile<2, 2, 1, 1, 64, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
-                                                     p.beta, m0, n0, vec, smem);
+    // interior tiles with K >= 64 a multiple of 8 and 16-byte aligned rows
+    // stage straight into LDS
+    const bool dma = vec && p.K >= 64 && p.K % 8 == 0;
+    if (shape == 0) {
+        if (dma && m0 + 128 <= p.M && n0 + 128 <= p.N)
+            bf16_tile_dma<2, 2, 2, 2, TA, TB>((const __bf16*)p.A, (const __bf16*)p.B, p.C, p.M, p.N, p.K, p.lda,
+                                              p.ldb, p.ldc, p.alpha, p.beta, m0, n0, smem);
+        else
+            bf16_tile<2, 2, 2, 2, 32, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc,
+                                                         p.alpha, p.beta, m0, n0, vec, smem);
+    } else {
+        if (dma && m0 + 64 <= p.M && n0 + 64 <= p.N)
+            bf16_tile_dma<2, 2, 1, 1, TA, TB>((const __bf16*)p.A, (const __bf16*)p.B, p.C, p.M, p.N, p.K, p.lda,
+                                              p.ldb, p.ldc, p.alpha, p.beta, m0, n0, smem);
+        else
+            bf16_tile<2, 2, 1, 1, 64, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc,
+                                                         p.alpha, p.beta, m0, n0, vec, smem);
+    }
 }
 
 // LDS of the tile buffers (the larger of the two shapes); the group's next
@@ -54,11 +70,15 @@ template <int BF16>
 constexpr size_t grouped_tile_bytes() {
     constexpr size_t a = BF16 ? Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM : SgemmShape<2, 2, 2, 2, 32, 32>::SMEM;
     constexpr size_t b = BF16 ? Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM : SgemmShape<2, 2, 1, 1, 32, 32>::SMEM;
-    return ((a > b ? a : b) + 15) / 16 * 16;
+    constexpr size_t c = BF16 ? Bf16DmaShape<2, 2, 2, 2>::SMEM : 0;  // direct-to-LDS stages of shape 0
+    constexpr size_t ab = a > b ? a : b;
+    return ((ab > c ? ab : c) + 15) / 16 * 16;
 }
 
+// Two work-groups per CU (LDS-limited): the register allocator is asked for
+// two waves per SIMD, which the union of all tile routines would exceed.
 template <int BF16>
-__global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupedProblemDev* __restrict__ probs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void grouped_gemm_kernel(const GroupedProblemDev* __restrict__ probs,
                                                            const int4* __restrict__ work, GroupedQueues q,
                                                            int* __restrict__ counter) {
     extern __shared__ __attribute__((aligned(16))) float gsmem[];

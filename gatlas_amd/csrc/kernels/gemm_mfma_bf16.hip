@@ -36,9 +36,9 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_simd<WM, WN, TM, TN, BK, MODE>
     int tmi, tni;
     mfma_tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, group_m, tmi, tni);
     if constexpr (MODE == MODE_BF16 && BK == 64) {
-        // interior tile, whole 64-deep K stages, 16-byte aligned rows: direct-to-LDS staging
+        // interior tile, K >= 64 a multiple of 8, 16-byte aligned rows: direct-to-LDS staging
         const int m0 = tmi * S::BM, n0 = tni * S::BN;
-        if (vec_ok && K >= 64 && K % 64 == 0 && m0 + S::BM <= M && n0 + S::BN <= N) {
+        if (vec_ok && K >= 64 && K % 8 == 0 && m0 + S::BM <= M && n0 + S::BN <= N) {
             bf16_tile_dma<WM, WN, TM, TN, TA, TB>((const __bf16*)A, (const __bf16*)B, C, M, N, K, lda, ldb, ldc,
                                                   alpha, beta, m0, n0, smemh);
             return;
