@@ -230,8 +230,9 @@ def sweep_bench(ctx, sizes=(1024, 2048, 4096, 8192)):
 
 
 def tuner_bench(ctx, sizes=(1024, 2048, 4096)):
-    """The autotuner's EM search over the MFMA tile table per size bucket:
-    default-tile TFLOP/s against tuned TFLOP/s, and the tile it installed."""
+    """The autotuner's search over the MFMA tile table per size bucket: the
+    default tile against the installed one, each timed in three alternating
+    rounds after the search (median), so clock drift cannot favour either."""
     import tempfile
 
     out = {}
@@ -242,16 +243,22 @@ def tuner_bench(ctx, sizes=(1024, 2048, 4096)):
                 ctx.fill_uniform(a, 0x6A7A5EED, i)
             # about 5 ms of launches at 100 TFLOP/s, 10..200 of them
             steps = max(10, min(200, int(5e-3 * 100e12 / (2.0 * n ** 3))))
-            ms0, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), steps, 5)
+            ctx.tuned_clear()
+            dcfg = ctx.tile_for(n, n, n)
             t0 = time.perf_counter()
             cfg, _ = ctx.tune_gemm(n, n, n, journal=os.path.join(d, "tune_%d.journal" % n))
             tune_s = time.perf_counter() - t0
-            ms1, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), steps, 5)
-            out["sgemm_%d" % n] = {"default_TFLOP/s": round(2.0 * n ** 3 / (ms0 * 1e-3) / 1e12, 2),
-                                   "tuned_TFLOP/s": round(2.0 * n ** 3 / (ms1 * 1e-3) / 1e12, 2),
-                                   "tile(wm,wn,tm,tn,bk,blk)": list(cfg), "search_s": round(tune_s, 2)}
+            ms = {"default": [], "tuned": []}
+            for _ in range(3):
+                for key, c in (("default", dcfg), ("tuned", cfg)):
+                    ms[key].append(timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n, cfg=c), steps, 5)[0])
+            tf = {k: 2.0 * n ** 3 / (sorted(v)[1] * 1e-3) / 1e12 for k, v in ms.items()}
+            out["sgemm_%d" % n] = {"default_TFLOP/s": round(tf["default"], 2), "tuned_TFLOP/s": round(tf["tuned"], 2),
+                                   "tile_default": list(dcfg), "tile(wm,wn,tm,tn,bk,blk)": list(cfg),
+                                   "search_s": round(tune_s, 2)}
             for a in (A, B, C):
                 a.free()
+    ctx.tuned_clear()
     return out
 
 

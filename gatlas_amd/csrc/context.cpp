@@ -185,7 +185,8 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
     }
     MfmaCfg big, mid, small, tiny;
     if (dtype == DT_BF16 || dtype == DT_BF16X3) {
-        big = dtype == DT_BF16 ? MfmaCfg{2, 4, 4, 2, 64, 32} : MfmaCfg{2, 4, 4, 2, 16, 32};  // 256 x 256, 8 waves
+        // 256 x 256, 8 waves; split-BF16x3 through pre-split planes
+        big = dtype == DT_BF16 ? MfmaCfg{2, 4, 4, 2, 64, 32} : x3_split_cfg_value();
         mid = MfmaCfg{2, 2, 2, 2, 32, 32};
         small = MfmaCfg{2, 2, 1, 1, 64, 32};
         tiny = MfmaCfg{1, 1, 2, 2, 64, 32};
@@ -211,12 +212,42 @@ int Context::gemm(int dtype, const GemmArgs& a, const MfmaCfg* cfg) {
     switch (dtype) {
         case DT_F32: return launch_gemm_mfma_f32(c, a, stream);
         case DT_F64: return launch_gemm_mfma_f64(c, a, stream);
-        case DT_BF16:
-        case DT_BF16X3: return launch_gemm_mfma_bf16(dtype, c, a, stream);
+        case DT_BF16: return launch_gemm_mfma_bf16(dtype, c, a, stream);
+        case DT_BF16X3:
+            if (x3_split_cfg(c)) return gemm_x3_split(a);
+            return launch_gemm_mfma_bf16(dtype, c, a, stream);
         case DT_FP8: return gemm_fp8(a, c);
     }
     set_error("gemm: precision form not available");
     return ST_UNSUPPORTED;
+}
+
+int Context::gemm_x3_split(const GemmArgs& a) {
+    if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return ST_OK;
+    if (a.K <= 0 || a.M > INT32_MAX || a.N > INT32_MAX || a.K > INT32_MAX) {
+        // nothing to split: the register-staged form handles the epilogue
+        return launch_gemm_mfma_bf16(DT_BF16X3, MfmaCfg{2, 4, 4, 2, 16, 32}, a, stream);
+    }
+    const X3SplitDims d = x3_split_dims(a.M, a.N, a.K);
+    void* w = nullptr;
+    int rc = workspace(d.bytes_a + d.bytes_b, &w);
+    if (rc) return rc;
+    unsigned char* base = (unsigned char*)w;
+    // op(A)[i][k] = A[i*lda + k] or A[k*lda + i]; op(B)^T[j][k] = B[k*ldb + j] or B[j*ldb + k]
+    const int64_t ars = a.transA ? 1 : a.lda, acs = a.transA ? a.lda : 1;
+    const int64_t brs = a.transB ? a.ldb : 1, bcs = a.transB ? 1 : a.ldb;
+    for (int b = 0; b < a.batch; ++b) {
+        const float* A = (const float*)a.A + (int64_t)b * a.strideA;
+        const float* B = (const float*)a.B + (int64_t)b * a.strideB;
+        float* C = (float*)a.C + (int64_t)b * a.strideC;
+        rc = launch_x3_split(A, ars, acs, (int)a.M, (int)a.K, d.mpad, d.kpad, base, stream);
+        if (!rc) rc = launch_x3_split(B, brs, bcs, (int)a.N, (int)a.K, d.npad, d.kpad, base + d.bytes_a, stream);
+        if (!rc)
+            rc = launch_x3_split_gemm(d, base, base + d.bytes_a, C, (int)a.M, (int)a.N, a.ldc, (float)a.alpha,
+                                      (float)a.beta, stream);
+        if (rc) return rc;
+    }
+    return ST_OK;
 }
 
 int Context::gemm_fp8(const GemmArgs& a, const MfmaCfg& c) {

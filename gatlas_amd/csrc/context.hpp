@@ -65,6 +65,7 @@ public:
 
 private:
     int gemm_fp8(const GemmArgs& a, const MfmaCfg& c);
+    int gemm_x3_split(const GemmArgs& a);
     std::unordered_map<void*, size_t> owned_;
     void* work_ = nullptr;
     size_t work_bytes_ = 0;

@@ -108,6 +108,24 @@ MfmaCfg mfma_bf16_cfg(int dtype, int i);
 bool mfma_bf16_has(int dtype, const MfmaCfg& cfg);
 int launch_convert_f32_bf16(const float* x, void* y, size_t n, hipStream_t st);
 
+// Split-BF16x3 through pre-split planes (gemm_x3_split.hip): op(A) and
+// op(B)^T are split once into hi/lo bf16 planes in a workspace (rows padded
+// to whole 256-row tiles, k to a multiple of 32), then a direct-to-LDS GEMM
+// runs on the planes.  The tile table lists it as the split-BF16x3
+// configuration (2, 4, 4, 2, 32, 32).
+struct X3SplitDims {
+    int64_t mpad = 0, npad = 0, kpad = 0;
+    size_t bytes_a = 0, bytes_b = 0;  // workspace of each plane pair
+};
+bool x3_split_cfg(const MfmaCfg& c);
+MfmaCfg x3_split_cfg_value();
+X3SplitDims x3_split_dims(int64_t M, int64_t N, int64_t K);
+// element (r, k) of the rows x K source is src[r*rs + k*cs]
+int launch_x3_split(const float* src, int64_t rs, int64_t cs, int rows, int K, int64_t rows_pad, int64_t kpad,
+                    void* dst, hipStream_t st);
+int launch_x3_split_gemm(const X3SplitDims& d, const void* As, const void* Bs, float* C, int M, int N,
+                         int64_t ldc, float alpha, float beta, hipStream_t st);
+
 // FP8 e4m3 (gemm_fp8.hip): block quantisation of a rows x cols operand whose
 // element (r, c) is src[r*rs + c*cs] into row-major bytes + block scales,
 // and the scaled GEMM (A8 = op(A) as M x K, B8 = op(B)^T as N x K).

@@ -75,17 +75,21 @@ static const Bf16CfgEntry kBf16Cfgs[] = {
 
 static int form_bit(int dtype) { return dtype == DT_BF16X3 ? 2 : 1; }
 
+// The split-BF16x3 table ends with the split-plane configuration
+// (gemm_x3_split.hip), which runs through the context (it needs a workspace).
 int mfma_bf16_num_cfgs(int dtype) {
     int n = 0;
     for (const Bf16CfgEntry& e : kBf16Cfgs) n += (e.forms & form_bit(dtype)) != 0;
-    return n;
+    return n + (dtype == DT_BF16X3 ? 1 : 0);
 }
 MfmaCfg mfma_bf16_cfg(int dtype, int i) {
     for (const Bf16CfgEntry& e : kBf16Cfgs)
         if ((e.forms & form_bit(dtype)) && i-- == 0) return e.cfg;
+    if (dtype == DT_BF16X3 && i == 0) return x3_split_cfg_value();
     return MfmaCfg{0, 0, 0, 0, 0, 0};
 }
 bool mfma_bf16_has(int dtype, const MfmaCfg& c) {
+    if (dtype == DT_BF16X3 && x3_split_cfg(c)) return true;
     for (const Bf16CfgEntry& e : kBf16Cfgs) {
         const MfmaCfg& k = e.cfg;
         if ((e.forms & form_bit(dtype)) && k.waves_m == c.waves_m && k.waves_n == c.waves_n && k.tm == c.tm &&
@@ -127,6 +131,10 @@ int launch_gemm_mfma_bf16(int dtype, const MfmaCfg& c, const GemmArgs& a, hipStr
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return ST_OK;
     if (a.M > INT32_MAX || a.N > INT32_MAX || a.K > INT32_MAX) return ST_INVALID;
     if (dtype != DT_BF16 && dtype != DT_BF16X3) return ST_INVALID;
+    if (dtype == DT_BF16X3 && x3_split_cfg(c)) {
+        set_error("mfma bf16: the split-plane configuration runs through the context");
+        return ST_INVALID;
+    }
     if (!mfma_bf16_has(dtype, c)) {
         set_error("mfma bf16: configuration not instantiated for this form");
         return ST_UNSUPPORTED;

@@ -1,0 +1,199 @@
+// Split-BF16x3 GEMM through pre-split operand planes (gfx950).
+//
+// A bandwidth pass splits op(A) (M x K) and op(B)^T (N x K) -- each element
+// a into hi = bf16(a) and lo = bf16(a - hi), the rounding of the in-kernel
+// form (split_bf16) -- into bf16 planes laid out k-contiguous, every 32 k
+// values stored as [32 hi | 32 lo]: 128 bytes per row per 32-deep K stage.
+// Rows are padded with zeros to whole tiles and k to a multiple of 32, so
+// every tile of the GEMM is interior.  The GEMM stages both planes straight
+// into LDS (DmaOperand, k-contiguous, XOR-swizzled rows) and issues, per
+// 16-deep step and block pair, lo*hi, hi*lo, hi*hi into FP32 accumulators:
+// the products and the order of the register-staged split-BF16x3 tile, so
+// the two forms agree bit for bit (zero padding adds exact zeros).
+#include "bf16_dma_tile.hpp"
+
+namespace gat {
+
+constexpr int X3S_ROWS = 64;  // rows per work-group of the split pass
+
+// dst row r (r < gridDim.y * 64): for each 32-k block b, 32 hi then 32 lo;
+// src element (r, k) = src[r*rs + k*cs], zero outside rows x K.
+__global__ __launch_bounds__(256) void x3_split_kernel(const float* __restrict__ src, int64_t rs, int64_t cs,
+                                                      int rows, int K, int64_t ldd, __bf16* __restrict__ dst) {
+    __shared__ float tile[X3S_ROWS][33];
+    const int b = blockIdx.x, r0 = blockIdx.y * X3S_ROWS, tid = threadIdx.x;
+    const bool kfast = cs == 1;  // k-contiguous source: lanes walk k, else rows
+    for (int e = tid; e < X3S_ROWS * 32; e += 256) {
+        const int r = kfast ? e >> 5 : e & 63, k = kfast ? e & 31 : e >> 6;
+        const int gr = r0 + r, gk = b * 32 + k;
+        tile[r][k] = (gr < rows && gk < K) ? gload<float>(src + (int64_t)gr * rs + (int64_t)gk * cs) : 0.0f;
+    }
+    __syncthreads();
+    // work-item: row tid/4; part tid%4 = hi k 0-15, hi 16-31, lo 0-15, lo 16-31
+    const int r = tid >> 2, p = tid & 3, kb = 16 * (p & 1);
+    unsigned wv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        __bf16 h0, l0, h1, l1;
+        split_bf16(tile[r][kb + 2 * j], h0, l0);
+        split_bf16(tile[r][kb + 2 * j + 1], h1, l1);
+        wv[j] = p < 2 ? pack_bf16(bf16_bits(h0), bf16_bits(h1)) : pack_bf16(bf16_bits(l0), bf16_bits(l1));
+    }
+    __bf16* d = dst + (int64_t)(r0 + r) * ldd + b * 64 + (p < 2 ? 0 : 32) + kb;
+    gstore(d, u32x4{wv[0], wv[1], wv[2], wv[3]});
+    gstore(d + 8, u32x4{wv[4], wv[5], wv[6], wv[7]});
+}
+
+// The GEMM on split planes: As is Mp x 2*kpad, Bs is Np x 2*kpad (row
+// stride ld2 elements), Mp and Np whole tiles.  Stage t holds k values
+// 32t .. 32t+31 of both planes.
+template <int WM, int WN, int TM, int TN>
+__global__ __launch_bounds__(WM* WN * 64) __attribute__((amdgpu_waves_per_eu(2))) void x3_split_gemm_kernel(
+    const __bf16* __restrict__ As, const __bf16* __restrict__ Bs, float* __restrict__ C, int M, int N, int kpad,
+    int64_t ld2, int64_t ldc, float alpha, float beta, int tiles_m, int tiles_n, int group_m) {
+    constexpr int NW = WM * WN, BM = WM * TM * 32, BN = WN * TN * 32;
+    typedef DmaOperand<BM, true, NW> OpA;
+    typedef DmaOperand<BN, true, NW> OpB;
+    constexpr int STAGE = OpA::BYTES + OpB::BYTES;
+    extern __shared__ __attribute__((aligned(16))) __bf16 xsm[];
+    char* const sb = (char*)xsm;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w / WN, wn = w % WN;
+    int tmi, tni;
+    mfma_tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, group_m, tmi, tni);
+    const int m0 = tmi * BM, n0 = tni * BN;
+
+    f16v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+    // 16-deep step s of a stage: hi chunk 2s + lane/32, lo chunk 4 more
+    auto frags = [&](const char* st, int s) {
+        const int c = 2 * s + (lane >> 5);
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+            const int row = wm * TM * 32 + i * 32 + (lane & 31);
+            const u32x4 h = OpA::chunk(st, row, c), l = OpA::chunk(st, row, c + 4);
+            ah[i] = *(const bf16x8*)&h;
+            al[i] = *(const bf16x8*)&l;
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int row = wn * TN * 32 + j * 32 + (lane & 31);
+            const u32x4 h = OpB::chunk(st + OpA::BYTES, row, c), l = OpB::chunk(st + OpA::BYTES, row, c + 4);
+            bh[j] = *(const bf16x8*)&h;
+            bl[j] = *(const bf16x8*)&l;
+        }
+    };
+    auto mma3 = [&]() {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+    };
+
+    const int KT = kpad / 32;
+    OpA::issue(As, ld2, m0, 0, sb, w, lane);
+    OpB::issue(Bs, ld2, n0, 0, sb + OpA::BYTES, w, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int t = 0; t < KT; ++t) {
+        const char* cur = sb + (t & 1) * STAGE;
+        char* nxt = sb + ((t + 1) & 1) * STAGE;
+        const int tn = t + 1 < KT ? t + 1 : t;  // the last iteration fetches its own stage again
+        OpA::issue(As, ld2, m0, 64 * tn, nxt, w, lane);
+        OpB::issue(Bs, ld2, n0, 64 * tn, nxt + OpA::BYTES, w, lane);
+        __builtin_amdgcn_iglp_opt(0);
+        frags(cur, 0);
+        mma3();
+        frags(cur, 1);
+        mma3();
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * TM * 32 + i * 32 + mfma32_row(lane, r);
+                const int col = n0 + wn * TN * 32 + j * 32 + mfma32_col(lane);
+                if (row < M && col < N) {
+                    float* p = C + (int64_t)row * ldc + col;
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.0f) v += beta * gload<float>(p);
+                    gstore(p, v);
+                }
+            }
+}
+
+// The one split-plane configuration: 256 x 256 tiles of 8 waves (4 x 2
+// blocks each), 32 k values per stage; listed in the split-BF16x3 tile
+// table with bk = 32.
+static const MfmaCfg kX3Split{2, 4, 4, 2, 32, 32};
+
+bool x3_split_cfg(const MfmaCfg& c) {
+    return c.waves_m == kX3Split.waves_m && c.waves_n == kX3Split.waves_n && c.tm == kX3Split.tm &&
+           c.tn == kX3Split.tn && c.bk == kX3Split.bk && c.blk == kX3Split.blk;
+}
+MfmaCfg x3_split_cfg_value() { return kX3Split; }
+
+X3SplitDims x3_split_dims(int64_t M, int64_t N, int64_t K) {
+    constexpr int BM = 256, BN = 256;
+    X3SplitDims d;
+    d.mpad = (M + BM - 1) / BM * BM;
+    d.npad = (N + BN - 1) / BN * BN;
+    d.kpad = (K + 31) / 32 * 32;
+    d.bytes_a = (size_t)d.mpad * 2 * d.kpad * sizeof(__bf16);
+    d.bytes_b = (size_t)d.npad * 2 * d.kpad * sizeof(__bf16);
+    return d;
+}
+
+int launch_x3_split(const float* src, int64_t rs, int64_t cs, int rows, int K, int64_t rows_pad, int64_t kpad,
+                    void* dst, hipStream_t st) {
+    if (rows_pad % X3S_ROWS || kpad % 32 || rows_pad / X3S_ROWS > 65535) {
+        set_error("split-bf16x3: padded shape out of range");
+        return ST_INVALID;
+    }
+    hipLaunchKernelGGL(x3_split_kernel, dim3((unsigned)(kpad / 32), (unsigned)(rows_pad / X3S_ROWS)), dim3(256), 0,
+                       st, src, rs, cs, rows, K, 2 * kpad, (__bf16*)dst);
+    GAT_HIP_TRY(hipGetLastError());
+    return ST_OK;
+}
+
+int launch_x3_split_gemm(const X3SplitDims& d, const void* As, const void* Bs, float* C, int M, int N,
+                         int64_t ldc, float alpha, float beta, hipStream_t st) {
+    constexpr int WM = 2, WN = 4, TM = 4, TN = 2, BM = WM * TM * 32, BN = WN * TN * 32;
+    constexpr size_t smem = 2 * (size_t)(BM + BN) * 128;
+    if (d.kpad <= 0 || d.kpad > INT32_MAX) {
+        set_error("split-bf16x3: K out of range");
+        return ST_INVALID;
+    }
+    const int tiles_m = (int)(d.mpad / BM), tiles_n = (int)(d.npad / BN);
+    const void* f = (const void*)x3_split_gemm_kernel<WM, WN, TM, TN>;
+    GAT_HIP_TRY(allow_dynamic_lds(f, smem));
+    const int group_m = tiles_m < 8 ? tiles_m : 8;
+    hipLaunchKernelGGL((x3_split_gemm_kernel<WM, WN, TM, TN>), dim3((unsigned)(tiles_m * tiles_n)),
+                       dim3(WM * WN * 64), smem, st, (const __bf16*)As, (const __bf16*)Bs, C, M, N, (int)d.kpad,
+                       2 * d.kpad, ldc, alpha, beta, tiles_m, tiles_n, group_m);
+    GAT_HIP_TRY(hipGetLastError());
+    return ST_OK;
+}
+
+}  // namespace gat

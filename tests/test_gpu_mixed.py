@@ -158,6 +158,52 @@ def test_bf16_direct_lds_path(ctx, ta, tb, K):
             assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (cfg, ta, tb)
 
 
+SPLIT_X3 = (2, 4, 4, 2, 32, 32)  # split-plane form of split-BF16x3
+REG_X3 = (2, 4, 4, 2, 16, 32)    # register-staged form, same tile
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_bf16x3_split_planes_match_register_form(ctx, ta, tb):
+    """The split-plane form (operands split once into padded hi/lo planes,
+    direct-to-LDS GEMM) issues the register-staged form's products in the
+    same order: equal results, ragged M, N and K not a multiple of 32, in
+    the general form; both within 1e-5 of the FP64 product."""
+    M, N, K = 300, 520, 200
+    _say("bf16x3 split planes", ta, tb, M, N, K)
+    A, B, C0 = _operands(M, N, K, ta, tb, 37)
+    dA, dB = ctx.to_device(A), ctx.to_device(B)
+    d1, d2 = ctx.to_device(C0), ctx.to_device(C0)
+    ctx.gemm(dA, dB, d1, M, N, K, transA=ta, transB=tb, alpha=1.5, beta=-0.5, dtype=_capi.GAT_BF16X3, cfg=SPLIT_X3)
+    ctx.gemm(dA, dB, d2, M, N, K, transA=ta, transB=tb, alpha=1.5, beta=-0.5, dtype=_capi.GAT_BF16X3, cfg=REG_X3)
+    got, ref = d1.download(), d2.download()
+    C64, MAG = ora.gemm64(A, B, M, N, K, transA=ta, transB=tb, alpha=1.5, beta=-0.5, C0=C0)
+    cw, _, _ = ora.error_metrics(got, C64, MAG)
+    assert cw <= TOL, cw
+    assert np.array_equal(got, ref)
+
+
+def test_bf16x3_split_planes_strided_batch(ctx):
+    # packedCalc-style batch through the split-plane form (tuned for the
+    # bucket): every problem equals its own single launch
+    import ctypes
+    M, N, K, P = 260, 130, 96, 3
+    A = uniform_host(P * M * K, 38, 0)
+    B = uniform_host(P * K * N, 38, 1)
+    dA, dB, dC = ctx.to_device(A), ctx.to_device(B), ctx.empty(P * M * N)
+    _capi.check(ctx.lib.gat_tuned_set(ctx.h, _capi.GAT_BF16X3, 0, 0, M, N, K, ctypes.byref(_capi.TileCfg(*SPLIT_X3))),
+                "tuned_set")
+    try:
+        ctx.gemm_strided(dA, dB, dC, M, N, K, P, dtype=_capi.GAT_BF16X3)
+        got = dC.download().reshape(P, M * N)
+    finally:
+        ctx.tuned_clear()
+    for p in range(P):
+        one = ctx.empty(M * N)
+        ctx.gemm(ctx.to_device(A[p * M * K:(p + 1) * M * K]), ctx.to_device(B[p * K * N:(p + 1) * K * N]), one, M, N,
+                 K, dtype=_capi.GAT_BF16X3, cfg=SPLIT_X3)
+        assert np.array_equal(got[p], one.download()), p
+
+
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
 def test_bf16_inputs_exact_product(ctx, ta, tb):
     M, N, K = 384, 256, 512
