@@ -373,8 +373,9 @@ def main():
         tf_x3 = flops / (ms_x3 * 1e-3) / 1e12
         sec["bf16x3_8192"] = {"ms": round(ms_x3, 4), "TFLOP/s": round(tf_x3, 2),
                               "vs_fp32_path": round(tf_x3 / tflops, 3),
-                              "note": "FP32 inputs split to bf16 hi/lo in the kernel; 3 MFMA products; "
-                                      "TFLOP/s counted as 2MNK"}
+                              "tile(wm,wn,tm,tn,bk,blk)": list(ctx.tile_for(n, n, n, dtype=_capi.GAT_BF16X3)),
+                              "note": "FP32 inputs split into bf16 hi/lo planes by a split pass (included in "
+                                      "the time), 3 MFMA products per element pair; TFLOP/s counted as 2MNK"}
         a16, b16 = ctx.empty(n * n, np.uint16), ctx.empty(n * n, np.uint16)
         ctx.convert_bf16(A, a16)
         ctx.convert_bf16(B, b16)
