@@ -108,7 +108,10 @@ int gat_scale_columns(gat_ctx *ctx, float *dptr, size_t rows, size_t cols, uint6
 /* C = alpha*op(A)*op(B) + beta*C; beta == 0 never reads C.
  * dtype GAT_F32 / GAT_F64: A, B, C of that type.
  * dtype GAT_BF16: A, B are bf16 arrays, C is FP32.
- * dtype GAT_BF16X3: A, B, C FP32 (split inside the library).
+ * dtype GAT_BF16X3: A, B, C FP32 (split inside the library: while the
+ *   tiles are staged, or -- tile configuration (2,4,4,2,32,32), the
+ *   default for large problems -- by a split pass into hi/lo planes in the
+ *   context workspace; both give the same values).
  * dtype GAT_FP8: A, B, C FP32; A and B are block-quantised to e4m3 in the
  *   context workspace, then multiplied (see gat_gemm_fp8 for resident e4m3). */
 int gat_gemm(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, size_t N, size_t K, double alpha,
