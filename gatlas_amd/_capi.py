@@ -47,7 +47,8 @@ class ServeCfg(ctypes.Structure):
     _fields_ = [("rate", _c_dbl), ("duration", _c_dbl), ("gemm_fraction", _c_dbl), ("hbm_budget", _c_sz),
                 ("mode", _c_int), ("streams", _c_int), ("max_batch", _c_int), ("max_inflight", _c_int),
                 ("gemm_min", _c_int), ("gemm_max", _c_int), ("gemv_min", _c_int), ("gemv_max", _c_int),
-                ("keep", _c_int), ("keep_max", _c_int), ("seed", ctypes.c_ulonglong)]
+                ("keep", _c_int), ("keep_max", _c_int), ("seed", ctypes.c_ulonglong),
+                ("gemm_dtype", _c_int)]
 
 
 class ServeStats(ctypes.Structure):

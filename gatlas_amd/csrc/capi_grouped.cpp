@@ -73,8 +73,8 @@ int gat_grouped_tune(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t
         set_error("gat_grouped_tune: null context or descriptors");
         return GAT_ERR_INVALID;
     }
-    if (dtype != GAT_F32 && dtype != GAT_BF16) {
-        set_error("gat_grouped_tune: dtype must be f32 or bf16");
+    if (dtype != GAT_F32 && dtype != GAT_BF16 && dtype != GAT_BF16X3) {
+        set_error("gat_grouped_tune: dtype must be f32, bf16 or bf16x3");
         return GAT_ERR_INVALID;
     }
     Context& c = ctx->impl;

@@ -78,6 +78,7 @@ void gat_serve_cfg_default(gat_serve_cfg* cfg) {
     cfg->keep = 0;
     cfg->keep_max = 0;
     cfg->seed = 1;
+    cfg->gemm_dtype = GAT_F32;
 }
 
 long gat_serve_trace(const gat_serve_cfg* cfg, gat_serve_arrival* out, size_t cap) {

@@ -16,8 +16,18 @@ GroupedPlan::~GroupedPlan() {
 }
 
 MfmaCfg GroupedPlan::shape_cfg(int dtype, int shape) {
-    if (dtype == DT_BF16) return shape == 0 ? MfmaCfg{2, 2, 2, 2, 32, 32} : MfmaCfg{2, 2, 1, 1, 64, 32};
+    if (dtype == DT_BF16 || dtype == DT_BF16X3)
+        return shape == 0 ? MfmaCfg{2, 2, 2, 2, 32, 32} : MfmaCfg{2, 2, 1, 1, 64, 32};
     return shape == 0 ? MfmaCfg{2, 2, 2, 2, 32, 32} : MfmaCfg{2, 2, 1, 1, 32, 32};
+}
+
+int GroupedPlan::form_of(int dtype) {
+    switch (dtype) {
+        case DT_F32: return GP_FORM_F32;
+        case DT_BF16: return GP_FORM_BF16;
+        case DT_BF16X3: return GP_FORM_X3;
+        default: return -1;
+    }
 }
 
 int GroupedPlan::choose_shape(const Context& ctx, int dtype, const GroupedDesc& d) {
@@ -36,11 +46,11 @@ int GroupedPlan::choose_shape(const Context& ctx, int dtype, const GroupedDesc& 
 int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* d, size_t n,
                               std::vector<GroupedProblemDev>& probs, std::vector<int4>& work, GroupedQueues& q,
                               std::vector<int>& shapes, double& flops, const int* force_shapes) {
-    if (dtype != DT_F32 && dtype != DT_BF16) {
-        set_error("grouped gemm: dtype must be f32 or bf16");
+    if (form_of(dtype) < 0) {
+        set_error("grouped gemm: dtype must be f32, bf16 or bf16x3");
         return ST_INVALID;
     }
-    const bool bf16 = dtype == DT_BF16;
+    const bool bf16 = dtype == DT_BF16;  // bf16 operands (x3 takes FP32 operands)
     probs.resize(n);
     shapes.assign(n, 0);
     flops = 0;
@@ -124,7 +134,7 @@ int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n,
     int rc = build_tables(ctx, dtype, d, n, probs, work, queues_, shapes_, flops_, force_shapes);
     if (rc) return rc;
     ctx_ = &ctx;
-    bf16_ = dtype == DT_BF16;
+    form_ = form_of(dtype);
     ctx.make_current();
     nwork_ = work.size();
     const size_t pbytes = std::max<size_t>(1, probs.size()) * sizeof(GroupedProblemDev);
@@ -144,10 +154,10 @@ int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n,
     // two resident work-groups per CU (LDS-limited), never more than the tiles
     grid_ = std::max(1, std::min<int>(2 * (ctx.num_cu > 0 ? ctx.num_cu : 256), (int)std::max<size_t>(1, nwork_)));
     if (use_graph) {
-        rc = grouped_gemm_prepare(bf16_);  // attribute calls stay outside the capture
+        rc = grouped_gemm_prepare(form_);  // attribute calls stay outside the capture
         if (rc) return rc;
         GAT_HIP_TRY(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
-        rc = launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, queues_,
+        rc = launch_grouped_gemm(form_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, queues_,
                                  d_counter_, grid_, ctx.stream);
         hipError_t e = hipStreamEndCapture(ctx.stream, &graph_);
         if (rc) return rc;
@@ -163,7 +173,7 @@ int GroupedPlan::run() {
         GAT_HIP_TRY(hipGraphLaunch(exec_, ctx_->stream));
         return ST_OK;
     }
-    return launch_grouped_gemm(bf16_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, queues_, d_counter_,
+    return launch_grouped_gemm(form_, (const GroupedProblemDev*)d_probs_, (const int4*)d_work_, queues_, d_counter_,
                                grid_, ctx_->stream);
 }
 

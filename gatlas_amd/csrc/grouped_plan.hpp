@@ -26,7 +26,8 @@ struct GroupedDesc {
 class GroupedPlan {
 public:
     ~GroupedPlan();
-    // dtype DT_F32 (FP32 operands) or DT_BF16 (bf16 operands), FP32 output.
+    // dtype DT_F32 (FP32 operands), DT_BF16 (bf16 operands) or DT_BF16X3
+    // (FP32 operands, split-BF16x3 products); FP32 output.
     // force_shapes (optional): the tile shape of every problem
     int create(Context& ctx, int dtype, const GroupedDesc* d, size_t n, bool use_graph,
                const int* force_shapes = nullptr);
@@ -41,6 +42,8 @@ public:
     // when one exists, else a size rule
     static int choose_shape(const Context& ctx, int dtype, const GroupedDesc& d);
     static MfmaCfg shape_cfg(int dtype, int shape);
+    // grouped-kernel precision form of a dtype (-1: not supported)
+    static int form_of(int dtype);
     // Device descriptors and the cost-ordered (problem, m0, n0) tile list
     // for n problems; also the tile shape per problem and the total flops.
     static int build_tables(const Context& ctx, int dtype, const GroupedDesc* d, size_t n,
@@ -49,7 +52,7 @@ public:
 
 private:
     Context* ctx_ = nullptr;
-    int bf16_ = 0;
+    int form_ = GP_FORM_F32;
     void* d_probs_ = nullptr;
     void* d_work_ = nullptr;
     int* d_counter_ = nullptr;

@@ -20,7 +20,10 @@
 //              1 = 64 x 64 (2 x 2 waves of one 32x32 block).
 // BF16 interior tiles of problems with K >= 64 a multiple of 8 and 16-byte
 // aligned rows stage straight into LDS (bf16_dma_tile.hpp); the others, and
-// every FP32 tile, stage through registers.
+// every FP32 and split-BF16x3 tile, stage through registers.  The
+// split-BF16x3 form splits FP32 operands into bf16 hi/lo planes while a
+// stage is stored to LDS (bf16_tile.hpp, MODE_X3): the same routine and
+// K depth as the single-problem x3 kernel of that tile, so the bits agree.
 #include "bf16_dma_tile.hpp"
 #include "bf16_tile.hpp"
 #include "grouped.hpp"
@@ -63,26 +66,65 @@ __device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int 
     }
 }
 
-// LDS of the tile buffers (the larger of the two shapes); the group's next
-// work index follows them.  The kernel declares no static LDS, so the
-// dynamic block -- and every 16-byte tile access in it -- starts aligned.
-template <int BF16>
+template <bool TA, bool TB>
+__device__ __forceinline__ void x3_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0,
+                                              __bf16* smem) {
+    const bool vec = (p.flags & GP_VEC) != 0;
+    if (shape == 0)
+        bf16_tile<2, 2, 2, 2, 32, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+                                                   p.beta, m0, n0, vec, smem);
+    else
+        bf16_tile<2, 2, 1, 1, 64, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+                                                   p.beta, m0, n0, vec, smem);
+}
+
+// LDS of the tile buffers (the larger of the two shapes).  The kernel
+// declares no static LDS, so the dynamic block -- and every 16-byte tile
+// access in it -- starts aligned.
+template <int FORM>
 constexpr size_t grouped_tile_bytes() {
-    constexpr size_t a = BF16 ? Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM : SgemmShape<2, 2, 2, 2, 32, 32>::SMEM;
-    constexpr size_t b = BF16 ? Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM : SgemmShape<2, 2, 1, 1, 32, 32>::SMEM;
-    constexpr size_t c = BF16 ? Bf16DmaShape<2, 2, 2, 2>::SMEM : 0;  // direct-to-LDS stages of shape 0
+    typedef Bf16Shape<2, 2, 2, 2, 32, MODE_X3> X0;
+    typedef Bf16Shape<2, 2, 1, 1, 64, MODE_X3> X1;
+    constexpr size_t a = FORM == GP_FORM_F32    ? SgemmShape<2, 2, 2, 2, 32, 32>::SMEM
+                         : FORM == GP_FORM_BF16 ? Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM
+                                                : X0::SMEM;
+    constexpr size_t b = FORM == GP_FORM_F32    ? SgemmShape<2, 2, 1, 1, 32, 32>::SMEM
+                         : FORM == GP_FORM_BF16 ? Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM
+                                                : X1::SMEM;
+    // direct-to-LDS stages of shape 0
+    constexpr size_t c = FORM == GP_FORM_BF16 ? Bf16DmaShape<2, 2, 2, 2>::SMEM : 0;
     constexpr size_t ab = a > b ? a : b;
     return ((ab > c ? ab : c) + 15) / 16 * 16;
 }
 
+// Byte offset of the group's next work index in LDS.  FP32 and BF16: a word
+// after the tile buffers.  Split-BF16x3: the two 80 KiB buffers of the 128 x
+// 128 tile fill the 160 KiB of a CU at two groups per CU, so the index sits
+// in the last 4 bytes of the block instead -- k columns BK..BK+7 of the last
+// row of the last plane are row padding that the x3 tile routine never reads
+// or writes (its loads and stores cover k < BK), and the 64 x 64 x3 tile's
+// buffers end before it.
+template <int FORM>
+constexpr size_t grouped_next_offset() {
+    if constexpr (FORM == GP_FORM_X3) {
+        typedef Bf16Shape<2, 2, 2, 2, 32, MODE_X3> X0;
+        typedef Bf16Shape<2, 2, 1, 1, 64, MODE_X3> X1;
+        static_assert(X0::SMEM % 16 == 0 && X0::SMEM >= X1::SMEM + 16, "64x64 x3 buffers must end before the slot");
+        static_assert(2 * X0::SMEM <= 163840, "two 128x128 x3 groups must fit one CU's LDS");
+        return X0::SMEM - 4;  // padding column BK+6..BK+7 (bf16) of the last staged row
+    } else {
+        return grouped_tile_bytes<FORM>();
+    }
+}
+
 // Two work-groups per CU (LDS-limited): the register allocator is asked for
 // two waves per SIMD, which the union of all tile routines would exceed.
-template <int BF16>
+template <int FORM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void grouped_gemm_kernel(const GroupedProblemDev* __restrict__ probs,
                                                            const int4* __restrict__ work, GroupedQueues q,
                                                            int* __restrict__ counter) {
     extern __shared__ __attribute__((aligned(16))) float gsmem[];
-    int& next = *(int*)((char*)gsmem + grouped_tile_bytes<BF16>());
+    int& next = *(int*)((char*)gsmem + grouped_next_offset<FORM>());
     const int home = (int)(blockIdx.x % GP_XCDS);
     int qi = 0;  // queues tried so far: the home queue first, then the others in turn
     for (;;) {
@@ -110,17 +152,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
         const GroupedProblemDev p = probs[t.x];
         const int shape = (p.flags >> GP_SHAPE_SHIFT) & 15;
         const int tr = p.flags & (GP_TA | GP_TB);
-        if (!BF16) {
+        if constexpr (FORM == GP_FORM_F32) {
             if (tr == 0) f32_tile_shape<false, false>(p, shape, t.y, t.z, gsmem);
             else if (tr == GP_TA) f32_tile_shape<true, false>(p, shape, t.y, t.z, gsmem);
             else if (tr == GP_TB) f32_tile_shape<false, true>(p, shape, t.y, t.z, gsmem);
             else f32_tile_shape<true, true>(p, shape, t.y, t.z, gsmem);
-        } else {
+        } else if constexpr (FORM == GP_FORM_BF16) {
             __bf16* hs = (__bf16*)gsmem;
             if (tr == 0) bf16_tile_shape<false, false>(p, shape, t.y, t.z, hs);
             else if (tr == GP_TA) bf16_tile_shape<true, false>(p, shape, t.y, t.z, hs);
             else if (tr == GP_TB) bf16_tile_shape<false, true>(p, shape, t.y, t.z, hs);
             else bf16_tile_shape<true, true>(p, shape, t.y, t.z, hs);
+        } else {
+            __bf16* hs = (__bf16*)gsmem;
+            if (tr == 0) x3_tile_shape<false, false>(p, shape, t.y, t.z, hs);
+            else if (tr == GP_TA) x3_tile_shape<true, false>(p, shape, t.y, t.z, hs);
+            else if (tr == GP_TB) x3_tile_shape<false, true>(p, shape, t.y, t.z, hs);
+            else x3_tile_shape<true, true>(p, shape, t.y, t.z, hs);
         }
     }
     if (threadIdx.x == 0) {
@@ -131,26 +179,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
     }
 }
 
-size_t grouped_smem_bytes(int bf16) {
-    return (bf16 ? grouped_tile_bytes<1>() : grouped_tile_bytes<0>()) + 16;
+size_t grouped_smem_bytes(int form) {
+    switch (form) {
+        case GP_FORM_F32: return grouped_next_offset<GP_FORM_F32>() + 16;
+        case GP_FORM_BF16: return grouped_next_offset<GP_FORM_BF16>() + 16;
+        default: return grouped_tile_bytes<GP_FORM_X3>();  // the index lives inside the tile block
+    }
 }
 
-int grouped_gemm_prepare(int bf16) {
-    const size_t smem = grouped_smem_bytes(bf16);
-    const void* f = bf16 ? (const void*)grouped_gemm_kernel<1> : (const void*)grouped_gemm_kernel<0>;
-    if (smem > 65536) GAT_HIP_TRY(allow_dynamic_lds(f, smem));
+static const void* grouped_kernel_fn(int form) {
+    switch (form) {
+        case GP_FORM_F32: return (const void*)grouped_gemm_kernel<GP_FORM_F32>;
+        case GP_FORM_BF16: return (const void*)grouped_gemm_kernel<GP_FORM_BF16>;
+        default: return (const void*)grouped_gemm_kernel<GP_FORM_X3>;
+    }
+}
+
+int grouped_gemm_prepare(int form) {
+    if (form < GP_FORM_F32 || form > GP_FORM_X3) return ST_INVALID;
+    const size_t smem = grouped_smem_bytes(form);
+    if (smem > 65536) GAT_HIP_TRY(allow_dynamic_lds(grouped_kernel_fn(form), smem));
     return ST_OK;
 }
 
-int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* work, const GroupedQueues& q,
+int launch_grouped_gemm(int form, const GroupedProblemDev* probs, const int4* work, const GroupedQueues& q,
                         int* counter, int grid, hipStream_t st) {
-    const size_t smem = grouped_smem_bytes(bf16);
-    const int rc = grouped_gemm_prepare(bf16);
+    const int rc = grouped_gemm_prepare(form);
     if (rc) return rc;
-    if (bf16)
-        hipLaunchKernelGGL(grouped_gemm_kernel<1>, dim3(grid), dim3(256), smem, st, probs, work, q, counter);
+    const size_t smem = grouped_smem_bytes(form);
+    if (form == GP_FORM_F32)
+        hipLaunchKernelGGL(grouped_gemm_kernel<GP_FORM_F32>, dim3(grid), dim3(256), smem, st, probs, work, q, counter);
+    else if (form == GP_FORM_BF16)
+        hipLaunchKernelGGL(grouped_gemm_kernel<GP_FORM_BF16>, dim3(grid), dim3(256), smem, st, probs, work, q,
+                           counter);
     else
-        hipLaunchKernelGGL(grouped_gemm_kernel<0>, dim3(grid), dim3(256), smem, st, probs, work, q, counter);
+        hipLaunchKernelGGL(grouped_gemm_kernel<GP_FORM_X3>, dim3(grid), dim3(256), smem, st, probs, work, q, counter);
     GAT_HIP_TRY(hipGetLastError());
     return ST_OK;
 }

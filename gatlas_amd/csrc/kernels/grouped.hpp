@@ -30,14 +30,21 @@ struct GroupedQueues {
     int off[GP_XCDS + 1];
 };
 
-size_t grouped_smem_bytes(int bf16);
+// Precision forms of a grouped launch (every problem of one launch shares it)
+enum : int {
+    GP_FORM_F32 = 0,   // FP32 operands, FP32 matrix instructions
+    GP_FORM_BF16 = 1,  // bf16 operands, FP32 accumulation
+    GP_FORM_X3 = 2     // FP32 operands split into bf16 hi/lo while staged, 3 products
+};
+
+size_t grouped_smem_bytes(int form);
 // Sets the kernel's dynamic LDS limit (once); needed before graph capture.
-int grouped_gemm_prepare(int bf16);
+int grouped_gemm_prepare(int form);
 // `work` holds int4 entries {problem, m0, n0, 0} in per-XCD queues `q`;
 // `counter` points to GP_XCDS + 1 ints (one cursor per queue, one count of
 // finished groups) that are zero before the first launch and zero again
 // after every launch completes.
-int launch_grouped_gemm(int bf16, const GroupedProblemDev* probs, const int4* work, const GroupedQueues& q,
+int launch_grouped_gemm(int form, const GroupedProblemDev* probs, const int4* work, const GroupedQueues& q,
                         int* counter, int grid, hipStream_t st);
 
 }  // namespace gat

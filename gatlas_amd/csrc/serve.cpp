@@ -159,14 +159,35 @@ Server::~Server() {
     if (arena_base_) hipFree(arena_base_);
 }
 
-MfmaCfg Server::gemm_cfg(const ServeReq& r) const { return GroupedPlan::shape_cfg(DT_F32, r.shape); }
+MfmaCfg Server::gemm_cfg(const ServeReq& r) const { return GroupedPlan::shape_cfg(cfg_.gemm_dtype, r.shape); }
+
+// One GEMM request alone: the single-problem kernel of the request's
+// precision form with the tile configuration of its grouped tile shape, so
+// the bits equal those of the grouped launch.
+int Server::launch_gemm_one(const ServeReq& r, float* C, hipStream_t st) const {
+    GemmArgs a;
+    a.transA = r.ta;
+    a.transB = r.tb;
+    a.M = r.M;
+    a.N = r.N;
+    a.K = r.K;
+    a.A = pool_ptr(r.aoff);
+    a.lda = r.lda;
+    a.B = pool_ptr(r.boff);
+    a.ldb = r.ldb;
+    a.C = C;
+    a.ldc = r.N;
+    if (cfg_.gemm_dtype == DT_BF16X3) return launch_gemm_mfma_bf16(DT_BF16X3, gemm_cfg(r), a, st);
+    return launch_gemm_mfma_f32(gemm_cfg(r), a, st);
+}
 
 int serve_check_cfg(const gat_serve_cfg& cfg) {
     if (!(cfg.rate > 0) || !(cfg.duration > 0) || cfg.gemm_fraction < 0 || cfg.gemm_fraction > 1 ||
         cfg.streams < 1 || cfg.streams > 16 || cfg.max_batch < 1 || cfg.max_batch > 4096 || cfg.max_inflight < 1 ||
         cfg.gemm_min < 16 || cfg.gemm_max < cfg.gemm_min || cfg.gemm_max > 16384 || cfg.gemv_min < 16 ||
         cfg.gemv_max < cfg.gemv_min || cfg.gemv_max > 65536 || cfg.hbm_budget < (1u << 20) ||
-        (cfg.mode != GAT_SERVE_BATCHED && cfg.mode != GAT_SERVE_LOOP)) {
+        (cfg.mode != GAT_SERVE_BATCHED && cfg.mode != GAT_SERVE_LOOP) ||
+        (cfg.gemm_dtype != GAT_F32 && cfg.gemm_dtype != GAT_BF16X3)) {
         set_error("serve: invalid configuration");
         return ST_INVALID;
     }
@@ -191,7 +212,7 @@ int Server::create(Context& ctx, const gat_serve_cfg& cfg) {
         }
         if (r.kind == 0) {
             GroupedDesc d{r.ta, r.tb, r.M, r.N, r.K, 1.0, nullptr, r.lda, nullptr, r.ldb, 0.0, nullptr, r.N};
-            r.shape = GroupedPlan::choose_shape(ctx, DT_F32, d);
+            r.shape = GroupedPlan::choose_shape(ctx, cfg_.gemm_dtype, d);
         }
     }
     GAT_HIP_TRY(hipMalloc((void**)&pool_, pool_elems_ * sizeof(float)));
@@ -239,19 +260,7 @@ int Server::launch_one(int id, int stream, Launch& L) {
     hipStream_t st = streams_[stream];
     int rc;
     if (r.kind == 0) {
-        GemmArgs a;
-        a.transA = r.ta;
-        a.transB = r.tb;
-        a.M = r.M;
-        a.N = r.N;
-        a.K = r.K;
-        a.A = pool_ptr(r.aoff);
-        a.lda = r.lda;
-        a.B = pool_ptr(r.boff);
-        a.ldb = r.ldb;
-        a.C = out_ptr(r);
-        a.ldc = r.N;
-        rc = launch_gemm_mfma_f32(gemm_cfg(r), a, st);
+        rc = launch_gemm_one(r, (float*)out_ptr(r), st);
     } else {
         const GemvCfg gc{256, GEMV_GROUPED_ROWS, GEMV_GROUPED_UNROLL};
         rc = launch_gemv_native(DT_F32, 0, (size_t)r.M, (size_t)r.N, 1.0, pool_ptr(r.aoff), (size_t)r.lda,
@@ -295,8 +304,8 @@ int Server::launch_batch(int kind, std::vector<int>& pend, int slot, Launch& L) 
         std::vector<int> used;
         double flops = 0;
         GroupedQueues q{};
-        int rc = GroupedPlan::build_tables(*ctx_, DT_F32, descs.data(), descs.size(), probs, work, q, used, flops,
-                                           shapes.data());
+        int rc = GroupedPlan::build_tables(*ctx_, cfg_.gemm_dtype, descs.data(), descs.size(), probs, work, q, used,
+                                           flops, shapes.data());
         if (rc) return rc;
         // [descriptors | tile list] in one staging buffer, one copy
         const size_t pbytes = probs.size() * sizeof(GroupedProblemDev);
@@ -306,7 +315,7 @@ int Server::launch_batch(int kind, std::vector<int>& pend, int slot, Launch& L) 
         GAT_HIP_TRY(hipMemcpyAsync(b.d_stage, b.h_stage, woff + work.size() * sizeof(int4), hipMemcpyHostToDevice,
                                    st));
         const int grid = (int)std::max<size_t>(1, std::min<size_t>(2 * (size_t)std::max(ctx_->num_cu, 1), work.size()));
-        rc = launch_grouped_gemm(0, (const GroupedProblemDev*)b.d_stage, (const int4*)(b.d_stage + woff), q,
+        rc = launch_grouped_gemm(GroupedPlan::form_of(cfg_.gemm_dtype), (const GroupedProblemDev*)b.d_stage, (const int4*)(b.d_stage + woff), q,
                                  b.d_counter, grid, st);
         if (rc) return rc;
         pend.erase(pend.begin(), pend.begin() + take);
@@ -529,19 +538,7 @@ int Server::check_alone(size_t* mismatches) {
         GAT_HIP_TRY(hipMalloc(&tmp, r.out_bytes));
         int rc;
         if (r.kind == 0) {
-            GemmArgs a;
-            a.transA = r.ta;
-            a.transB = r.tb;
-            a.M = r.M;
-            a.N = r.N;
-            a.K = r.K;
-            a.A = pool_ptr(r.aoff);
-            a.lda = r.lda;
-            a.B = pool_ptr(r.boff);
-            a.ldb = r.ldb;
-            a.C = tmp;
-            a.ldc = r.N;
-            rc = launch_gemm_mfma_f32(gemm_cfg(r), a, ctx.stream);
+            rc = launch_gemm_one(r, (float*)tmp, ctx.stream);
         } else {
             const GemvCfg gc{256, GEMV_GROUPED_ROWS, GEMV_GROUPED_UNROLL};
             rc = launch_gemv_native(DT_F32, 0, (size_t)r.M, (size_t)r.N, 1.0, pool_ptr(r.aoff), (size_t)r.lda,

@@ -80,6 +80,7 @@ private:
     float* pool_ptr(size_t off) const { return pool_ + off; }
     float* out_ptr(const ServeReq& r) const { return (float*)((char*)arena_base_ + r.out_off); }
     MfmaCfg gemm_cfg(const ServeReq& r) const;
+    int launch_gemm_one(const ServeReq& r, float* C, hipStream_t st) const;
     int launch_batch(int kind, std::vector<int>& pend, int slot, Launch& L);
     int launch_one(int id, int stream, Launch& L);
     hipEvent_t get_event();

@@ -177,8 +177,9 @@ int gat_axpy_compat(gat_ctx *ctx, int dtype, size_t M, size_t N, double alpha, c
 /* Grouped GEMM (no reference counterpart; the reference's only batching   */
 /* is packedCalc, same-shape problems in one kernel): many independent     */
 /* problems of any shapes/transposes in ONE persistent launch, optionally  */
-/* captured in a HIP graph.  dtype GAT_F32 (FP32 operands) or GAT_BF16     */
-/* (bf16 operands); C is FP32.                                             */
+/* captured in a HIP graph.  dtype GAT_F32 (FP32 operands), GAT_BF16      */
+/* (bf16 operands) or GAT_BF16X3 (FP32 operands split into bf16 hi/lo as   */
+/* they are staged, three products: FP32-level error); C is FP32.          */
 /* ---------------------------------------------------------------------- */
 typedef struct gat_gemm_desc {
     int transA, transB;
@@ -247,6 +248,8 @@ typedef struct gat_serve_cfg {
     int keep;                     /* keep every keep-th result (0: none) ...       */
     int keep_max;                 /* ... up to keep_max results                    */
     unsigned long long seed;
+    int gemm_dtype;               /* GEMM requests: GAT_F32 or GAT_BF16X3 (FP32    */
+                                  /* operands and results either way)             */
 } gat_serve_cfg;
 
 typedef struct gat_serve_stats {

@@ -2,7 +2,8 @@
 
   * every problem of a mixed group (ragged sizes, all transpose pairs, general
     alpha/beta, empty problems) against the FP64 oracle: componentwise
-    error <= 1e-5 (FP32) and <= 1e-5 against the bf16-rounded operands (BF16);
+    error <= 1e-5 (FP32 and split-BF16x3, against the FP32 operands) and
+    <= 1e-5 against the bf16-rounded operands (BF16);
   * each problem's result is bit-identical to a single launch of the same
     tile configuration (a problem computes the same alone as in the group);
   * a captured graph replayed several times, with eager work between the
@@ -21,7 +22,10 @@ from tests.helpers import bf16_bits_to_f32, uniform_host
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
 SHAPE_CFG = {_capi.GAT_F32: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 32, 32)],
-             _capi.GAT_BF16: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 64, 32)]}
+             _capi.GAT_BF16: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 64, 32)],
+             _capi.GAT_BF16X3: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 64, 32)]}
+FORMS = [_capi.GAT_F32, _capi.GAT_BF16, _capi.GAT_BF16X3]
+FORM_IDS = ["f32", "bf16", "bf16x3"]
 
 
 def _say(*a):
@@ -76,7 +80,7 @@ def _ragged_dims(rng, n, lo, hi):
     return dims
 
 
-@pytest.mark.parametrize("dtype", [_capi.GAT_F32, _capi.GAT_BF16], ids=["f32", "bf16"])
+@pytest.mark.parametrize("dtype", FORMS, ids=FORM_IDS)
 def test_grouped_mixed_against_oracle(ctx, dtype):
     rng = np.random.default_rng(7)
     dims = _ragged_dims(rng, 24, 1, 300)
@@ -90,7 +94,7 @@ def test_grouped_mixed_against_oracle(ctx, dtype):
     print("[grouped] dtype %d worst componentwise err %.3e, %s" % (dtype, worst, g.info()))
 
 
-@pytest.mark.parametrize("dtype", [_capi.GAT_F32, _capi.GAT_BF16], ids=["f32", "bf16"])
+@pytest.mark.parametrize("dtype", FORMS, ids=FORM_IDS)
 def test_grouped_equals_single_launch(ctx, dtype):
     rng = np.random.default_rng(8)
     dims = _ragged_dims(rng, 12, 16, 520)
@@ -154,7 +158,7 @@ def _workload_dims(rng, n):
     return [(int(a), int(b), int(c), int(x), int(y)) for (a, b, c), (x, y) in zip(v, t)]
 
 
-@pytest.mark.parametrize("dtype", [_capi.GAT_F32, _capi.GAT_BF16], ids=["f32", "bf16"])
+@pytest.mark.parametrize("dtype", FORMS, ids=FORM_IDS)
 def test_grouped_512_workload(ctx, dtype):
     rng = np.random.default_rng(0x67525550)
     dims = _workload_dims(rng, 512)

@@ -2,10 +2,11 @@
 
   * grouped GEMV: every problem bit-identical to its single gat_gemv launch
     and within 1e-5 (componentwise) of the FP64 oracle;
-  * serving (small trace): every request finishes, statistics are
-    consistent, each kept result is bit-identical to the same request run
-    alone, the batched and the per-request-loop modes give identical bits
-    for the same trace, and kept results match the FP64 oracle;
+  * serving (small trace, GEMM requests in FP32 and in split-BF16x3): every
+    request finishes, statistics are consistent, each kept result is
+    bit-identical to the same request run alone, the batched and the
+    per-request-loop modes give identical bits for the same trace, and kept
+    results match the FP64 oracle within 1e-5;
   * a budget just above the largest request still drains the trace.
 """
 import numpy as np
@@ -83,9 +84,11 @@ def _oracle_check(srv, kept):
         assert cw <= TOL, (r, cw)
 
 
-def test_serve_batched_vs_loop(ctx):
-    _say("serve small batched")
-    srv = Server(ctx, mode=_capi.GAT_SERVE_BATCHED, **SMALL)
+@pytest.mark.parametrize("gemm_dtype", [_capi.GAT_F32, _capi.GAT_BF16X3], ids=["f32", "bf16x3"])
+def test_serve_batched_vs_loop(ctx, gemm_dtype):
+    cfg = dict(SMALL, gemm_dtype=gemm_dtype)
+    _say("serve small batched", gemm_dtype)
+    srv = Server(ctx, mode=_capi.GAT_SERVE_BATCHED, **cfg)
     st = srv.run()
     print("[serve batched]", {k: round(v, 4) if isinstance(v, float) else v for k, v in st.items()}, flush=True)
     assert st["requests"] == st["gemm_requests"] + st["gemv_requests"] > 100
@@ -98,8 +101,8 @@ def test_serve_batched_vs_loop(ctx):
     _oracle_check(srv, kb)
     srv.close()
 
-    _say("serve small loop")
-    srv2 = Server(ctx, mode=_capi.GAT_SERVE_LOOP, **SMALL)
+    _say("serve small loop", gemm_dtype)
+    srv2 = Server(ctx, mode=_capi.GAT_SERVE_LOOP, **cfg)
     st2 = srv2.run()
     print("[serve loop]", {k: round(v, 4) if isinstance(v, float) else v for k, v in st2.items()}, flush=True)
     assert st2["requests"] == st["requests"] and st2["launches"] == st2["requests"]
