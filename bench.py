@@ -222,7 +222,7 @@ def sgemm_1024_bench(ctx, with_cpu):
     ctx.fill_uniform(B, 0x6A7A5EED, 1)
     ms, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n), 200, 5)
     out = {"gpu_ms": round(ms, 4), "gpu_TFLOP/s": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 2),
-           "tile(wm,wn,tm,tn,bk,blk)": list(ctx.tile_for(n, n, n))}
+           "tile(wm,wn,tm,tn,bk,blk,wk)": list(ctx.tile_for(n, n, n))}
     for a in (A, B, C):
         a.free()
     if with_cpu:
@@ -289,7 +289,7 @@ def tuner_bench(ctx, sizes=(1024, 2048, 4096)):
                     ms[key].append(timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n, cfg=c), steps, 5)[0])
             tf = {k: 2.0 * n ** 3 / (sorted(v)[1] * 1e-3) / 1e12 for k, v in ms.items()}
             out["sgemm_%d" % n] = {"default_TFLOP/s": round(tf["default"], 2), "tuned_TFLOP/s": round(tf["tuned"], 2),
-                                   "tile_default": list(dcfg), "tile(wm,wn,tm,tn,bk,blk)": list(cfg),
+                                   "tile_default": list(dcfg), "tile(wm,wn,tm,tn,bk,blk,wk)": list(cfg),
                                    "search_s": round(tune_s, 2)}
             for a in (A, B, C):
                 a.free()
@@ -394,7 +394,7 @@ def main():
         "data": "synthetic (U[-1,1] from the device generator, seed 0x6A7A5EED)",
         "config": {"workload": "SGEMM C = 1.5*A*B - 0.5*C, M=N=K=%d, FP32 row-major, no transpose" % n,
                    "M": n, "N": n, "K": n, "kernel": "MFMA FP32 (v_mfma_f32_32x32x2f32)",
-                   "tile(wm,wn,tm,tn,bk,blk)": list(ctx.tile_for(n, n, n)),
+                   "tile(wm,wn,tm,tn,bk,blk,wk)": list(ctx.tile_for(n, n, n)),
                    "pct_peak_fp32": round(100.0 * tflops / PEAK_FP32, 2),
                    "wall_s_timed_region": round(wall, 4)},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
@@ -408,7 +408,7 @@ def main():
         tf_x3 = flops / (ms_x3 * 1e-3) / 1e12
         sec["bf16x3_8192"] = {"ms": round(ms_x3, 4), "TFLOP/s": round(tf_x3, 2),
                               "vs_fp32_path": round(tf_x3 / tflops, 3),
-                              "tile(wm,wn,tm,tn,bk,blk)": list(ctx.tile_for(n, n, n, dtype=_capi.GAT_BF16X3)),
+                              "tile(wm,wn,tm,tn,bk,blk,wk)": list(ctx.tile_for(n, n, n, dtype=_capi.GAT_BF16X3)),
                               "note": "FP32 inputs split into bf16 hi/lo planes by a split pass (included in "
                                       "the time), 3 MFMA products per element pair; TFLOP/s counted as 2MNK"}
         a16, b16 = ctx.empty(n * n, np.uint16), ctx.empty(n * n, np.uint16)

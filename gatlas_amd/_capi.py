@@ -22,10 +22,11 @@ _c_u64 = ctypes.c_uint64
 
 class TileCfg(ctypes.Structure):
     _fields_ = [("waves_m", _c_int), ("waves_n", _c_int), ("tm", _c_int), ("tn", _c_int),
-                ("bk", _c_int), ("blk", _c_int)]
+                ("bk", _c_int), ("blk", _c_int), ("wk", _c_int)]
 
     def as_tuple(self):
-        return (self.waves_m, self.waves_n, self.tm, self.tn, self.bk, self.blk)
+        """(waves_m, waves_n, tm, tn, bk, blk, wk), wk normalised to >= 1."""
+        return (self.waves_m, self.waves_n, self.tm, self.tn, self.bk, self.blk, max(1, self.wk))
 
 
 class GemmDesc(ctypes.Structure):

@@ -232,7 +232,7 @@ int gat_gemm_cfg(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, size
     if (rc) return rc;
     ctx->impl.make_current();
     MfmaCfg c;
-    if (cfg) c = MfmaCfg{cfg->waves_m, cfg->waves_n, cfg->tm, cfg->tn, cfg->bk, cfg->blk};
+    if (cfg) c = from_tile_cfg(*cfg);
     return ctx->impl.gemm(dtype, a, cfg ? &c : nullptr);
 }
 
@@ -244,7 +244,7 @@ int gat_tile_cfg_get(int dtype, int i, gat_tile_cfg* out) {
         return GAT_ERR_INVALID;
     }
     MfmaCfg c = Context::cfg_at(dtype, i);
-    *out = gat_tile_cfg{c.waves_m, c.waves_n, c.tm, c.tn, c.bk, c.blk};
+    *out = to_tile_cfg(c);
     return GAT_OK;
 }
 

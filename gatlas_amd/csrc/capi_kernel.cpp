@@ -344,7 +344,8 @@ int gat_tune_gemm(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, siz
     if (best.empty()) return GAT_ERR_UNSUPPORTED;
     m->setParams(best);
     const std::vector<size_t> d = m->extraParamDetail();
-    const MfmaCfg c{(int)best[7], (int)best[8], (int)best[9], (int)best[10], kMfmaBK[d[0]], kMfmaBlk[d[1]]};
+    const MfmaCfg c{(int)best[7], (int)best[8], (int)best[9], (int)best[10], kMfmaBK[d[0]], kMfmaBlk[d[1]],
+                    kMfmaWK[d[2]]};
     GemmArgs a;
     a.transA = transA;
     a.transB = transB;
@@ -352,7 +353,7 @@ int gat_tune_gemm(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, siz
     a.N = (int64_t)N;
     a.K = (int64_t)K;
     ctx->impl.set_tuned(bucket_of(dtype, a), c);
-    if (best_out) *best_out = gat_tile_cfg{c.waves_m, c.waves_n, c.tm, c.tn, c.bk, c.blk};
+    if (best_out) *best_out = to_tile_cfg(c);
     if (gflops) {
         journal.loadMemo();
         const int us = journal.memoTime(*k, best, 0);
@@ -364,7 +365,7 @@ int gat_tune_gemm(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, siz
 int gat_tuned_set(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, size_t N, size_t K,
                   const gat_tile_cfg* cfg) {
     if (!ctx || !cfg) return bad("null context or config");
-    const MfmaCfg c{cfg->waves_m, cfg->waves_n, cfg->tm, cfg->tn, cfg->bk, cfg->blk};
+    const MfmaCfg c = from_tile_cfg(*cfg);
     if (!Context::has_cfg(dtype, c)) return bad("gat_tuned_set: configuration not instantiated");
     GemmArgs a;
     a.transA = transA;
@@ -387,7 +388,7 @@ int gat_tuned_get(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, siz
     MfmaCfg c;
     const bool tuned = ctx->impl.get_tuned(bucket_of(dtype, a), &c);
     if (!tuned) c = Context::default_cfg(dtype, a);
-    *out = gat_tile_cfg{c.waves_m, c.waves_n, c.tm, c.tn, c.bk, c.blk};
+    *out = to_tile_cfg(c);
     return tuned ? 1 : 0;
 }
 

@@ -154,6 +154,7 @@ MfmaCfg Context::cfg_at(int dtype, int i) {
 }
 
 bool Context::has_cfg(int dtype, const MfmaCfg& c) {
+    if (c.wk != 1 && dtype != DT_F32) return false;  // k-split wave groups: FP32 tiles only
     switch (dtype) {
         case DT_F32: return mfma_f32_has(c);
         case DT_F64: return mfma_f64_has(c);
@@ -167,16 +168,20 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
 // Untuned tile rule: the largest tile of the form that still gives at least
 // one work-group per CU of an MI355X (256 CUs), counting batched problems;
 // a problem too small to fill the chip with any tile gets the 64 x 64 tile
-// of four waves (one wave when a side is below 256).  At 1024^3, 2048^3 and
-// 4096^3 FP32 this picks the same tiles as an exhaustive timing of the table.
+// of four waves (one wave when a side is below 256).  FP32 tiles that give
+// fewer than two work-groups per CU take their k-split form (two or four
+// wave groups share each K stage), which keeps more than one wave per SIMD.
+// At 1024^3, 2048^3 and 4096^3 FP32 this picks the same tiles as an
+// exhaustive timing of the table.
 MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
     if (dtype == DT_F64) return MfmaCfg{2, 2, 2, 2, 16, 16};
     constexpr int64_t kFillTiles = 256;
     const int64_t batch = a.batch > 1 ? a.batch : 1;
-    auto fills = [&](const MfmaCfg& c) {
+    auto tiles = [&](const MfmaCfg& c) {
         const int64_t bm = (int64_t)c.waves_m * c.tm * c.blk, bn = (int64_t)c.waves_n * c.tn * c.blk;
-        return ((a.M + bm - 1) / bm) * ((a.N + bn - 1) / bn) * batch >= kFillTiles;
+        return ((a.M + bm - 1) / bm) * ((a.N + bn - 1) / bn) * batch;
     };
+    auto fills = [&](const MfmaCfg& c) { return tiles(c) >= kFillTiles; };
     const int64_t mn = a.M < a.N ? a.M : a.N;
     if (dtype == DT_FP8) {
         const MfmaCfg big{2, 4, 4, 2, 128, 32};  // 256 x 256, 8 waves
@@ -196,9 +201,11 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
         small = MfmaCfg{2, 2, 1, 1, 32, 32};
         tiny = MfmaCfg{1, 1, 2, 2, 32, 32};
     }
+    const bool split = dtype == DT_F32;
     if (fills(big)) return big;
-    if (fills(mid)) return mid;
-    if (fills(small) || mn >= 256) return small;
+    if (fills(mid)) return split && tiles(mid) < 2 * kFillTiles ? MfmaCfg{2, 2, 2, 2, 32, 32, 2} : mid;
+    if (fills(small) || mn >= 256)
+        return split && tiles(small) < 2 * kFillTiles ? MfmaCfg{2, 2, 1, 1, 64, 32, 4} : small;
     return tiny;
 }
 

@@ -12,6 +12,7 @@ namespace gat {
 
 const int kMfmaBK[4] = {8, 16, 32, 64};
 const int kMfmaBlk[2] = {32, 16};
+const int kMfmaWK[3] = {1, 2, 4};
 
 template <typename T>
 static const char* nameof();
@@ -222,6 +223,7 @@ public:
         dtype = dt;
         xp_.addDigit(4);  // LDS depth bk in kMfmaBK
         xp_.addDigit(2);  // block shape in kMfmaBlk
+        xp_.addDigit(3);  // wave groups sharing each K tile in kMfmaWK
     }
     double tolerance() const {
         switch (dtype) {
@@ -240,7 +242,8 @@ public:
     MfmaCfg cfg() const {
         const auto d = extraParamDetail();
         return MfmaCfg{(int)groupHeight(), (int)groupWidth(), (int)blockHeight(), (int)blockWidth(),
-                       kMfmaBK[d.empty() ? 0 : d[0]], kMfmaBlk[d.size() < 2 ? 0 : d[1]]};
+                       kMfmaBK[d.empty() ? 0 : d[0]], kMfmaBlk[d.size() < 2 ? 0 : d[1]],
+                       kMfmaWK[d.size() < 3 ? 0 : d[2]]};
     }
     bool validParams() const override {
         return packedCalc() > 0 && dimM() > 0 && dimN() > 0 && dimK() > 0 && extraParam() < totalVariations() &&
@@ -251,9 +254,11 @@ public:
         const size_t BM = (size_t)c.waves_m * c.tm * c.blk, BN = (size_t)c.waves_n * c.tn * c.blk;
         if (!BM || !BN) return {0, 0};
         const size_t tiles = ((dimM() + BM - 1) / BM) * ((dimN() + BN - 1) / BN);
-        return {tiles * (size_t)c.waves_m * c.waves_n * 64, packedCalc()};
+        return {tiles * (size_t)c.waves_m * c.waves_n * c.wk * 64, packedCalc()};
     }
-    std::vector<size_t> localWorkItems() const override { return {groupHeight() * groupWidth() * 64, 1}; }
+    std::vector<size_t> localWorkItems() const override {
+        return {groupHeight() * groupWidth() * (size_t)cfg().wk * 64, 1};
+    }
     void paranoidCheck() override { paranoid_ = true; }
     double lastCheckError() const override { return lastErr_; }
 
@@ -365,7 +370,9 @@ public:
         os << "// " << kernelName() << ": MFMA GEMM, " << c.waves_m << "x" << c.waves_n << " wavefronts, each "
            << c.tm << "x" << c.tn << " blocks of " << c.blk << "x" << c.blk << ", macro tile "
            << c.waves_m * c.tm * c.blk << " x " << c.waves_n * c.tn * c.blk << ", LDS depth " << c.bk
-           << " (double-buffered)\n//   M=" << dimM() << " N=" << dimN() << " K=" << dimK()
+           << " (double-buffered)";
+        if (c.wk > 1) os << ", each K tile split between " << c.wk << " wave groups";
+        os << "\n//   M=" << dimM() << " N=" << dimN() << " K=" << dimK()
            << " transA=" << transposeA() << " transB=" << transposeB() << " GEMM=" << generalizedMatmul()
            << " packed=" << packedCalc() << " instantiated=" << (validParams() ? "yes" : "no") << "\n";
         return os;

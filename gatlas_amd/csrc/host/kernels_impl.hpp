@@ -49,5 +49,6 @@ int kernel_dtype(KernelInterface* k);
 // MFMA digit tables (LDS depth, block shape).
 extern const int kMfmaBK[4];
 extern const int kMfmaBlk[2];
+extern const int kMfmaWK[3];
 
 }  // namespace gat

@@ -371,6 +371,10 @@ static int launch_fp8_one(const GemmArgs& a, hipStream_t st) {
 
 int launch_gemm_fp8(const MfmaCfg& c, const GemmArgs& a, hipStream_t st) {
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return ST_OK;
+    if (c.wk != 1) {
+        set_error("k-split wave groups exist for FP32 tiles only");
+        return ST_UNSUPPORTED;
+    }
     if (a.M > INT32_MAX || a.N > INT32_MAX || a.K > INT32_MAX) return ST_INVALID;
     if (a.transA || !a.transB || a.lda != a.K || a.ldb != a.K || !a.scaleA || !a.scaleB || a.batch != 1) {
         set_error("fp8 gemm: A must be op(A) as M x K and B op(B) as N x K (k-contiguous), with block scales");

@@ -129,6 +129,10 @@ static int launch_bf16_one(const GemmArgs& a, hipStream_t st) {
 
 int launch_gemm_mfma_bf16(int dtype, const MfmaCfg& c, const GemmArgs& a, hipStream_t st) {
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return ST_OK;
+    if (c.wk != 1) {
+        set_error("k-split wave groups exist for FP32 tiles only");
+        return ST_UNSUPPORTED;
+    }
     if (a.M > INT32_MAX || a.N > INT32_MAX || a.K > INT32_MAX) return ST_INVALID;
     if (dtype != DT_BF16 && dtype != DT_BF16X3) return ST_INVALID;
     if (dtype == DT_BF16X3 && x3_split_cfg(c)) {

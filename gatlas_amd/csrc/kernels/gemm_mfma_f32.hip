@@ -5,41 +5,47 @@
 
 namespace gat {
 
-template <int WM, int WN, int TM, int TN, int BK, int BLK, bool TA, bool TB>
-__global__ __launch_bounds__(WM* WN * 64) void sgemm_mfma_kernel(
+template <int WM, int WN, int TM, int TN, int BK, int BLK, int WK, bool TA, bool TB>
+__global__ __launch_bounds__(WM* WN * WK * 64) void sgemm_mfma_kernel(
     const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ C, int M, int N, int K,
     int64_t lda, int64_t ldb, int64_t ldc, int64_t sA, int64_t sB, int64_t sC, float alpha, float beta,
     int tiles_m, int tiles_n, int group_m, int vec_ok) {
-    typedef SgemmShape<WM, WN, TM, TN, BK, BLK> S;
+    typedef SgemmShape<WM, WN, TM, TN, BK, BLK, WK> S;
     extern __shared__ float smem[];
     A += (int64_t)blockIdx.y * sA;
     B += (int64_t)blockIdx.y * sB;
     C += (int64_t)blockIdx.y * sC;
     int tmi, tni;
     mfma_tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, group_m, tmi, tni);
-    sgemm_tile<WM, WN, TM, TN, BK, BLK, TA, TB>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, tmi * S::BM,
-                                                tni * S::BN, vec_ok != 0, smem);
+    sgemm_tile<WM, WN, TM, TN, BK, BLK, TA, TB, WK>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, tmi * S::BM,
+                                                    tni * S::BN, vec_ok != 0, smem);
 }
 
-// Instantiated configurations: (WM, WN, TM, TN, BK, BLK).
-#define GAT_F32_MFMA_CFGS(X) \
-    X(2, 2, 2, 2, 16, 32)    \
-    X(2, 2, 2, 2, 32, 32)    \
-    X(2, 2, 4, 2, 16, 32)    \
-    X(2, 2, 2, 4, 16, 32)    \
-    X(2, 2, 4, 4, 16, 32)    \
-    X(2, 2, 1, 1, 32, 32)    \
-    X(1, 1, 2, 2, 32, 32)    \
-    X(2, 2, 2, 2, 32, 16)    \
-    X(2, 2, 4, 4, 16, 16)    \
-    X(2, 2, 4, 4, 8, 32)     \
-    X(2, 4, 4, 2, 16, 32)    \
-    X(2, 4, 4, 2, 32, 32)    \
-    X(4, 2, 2, 4, 16, 32)    \
-    X(4, 4, 2, 2, 16, 32)
+// Instantiated configurations: (WM, WN, TM, TN, BK, BLK, WK).  The WK > 1
+// entries split each staged K tile between WK wave groups: small problems,
+// whose tiles barely cover the CUs, then keep several waves per SIMD.
+#define GAT_F32_MFMA_CFGS(X)  \
+    X(2, 2, 2, 2, 16, 32, 1)  \
+    X(2, 2, 2, 2, 32, 32, 1)  \
+    X(2, 2, 4, 2, 16, 32, 1)  \
+    X(2, 2, 2, 4, 16, 32, 1)  \
+    X(2, 2, 4, 4, 16, 32, 1)  \
+    X(2, 2, 1, 1, 32, 32, 1)  \
+    X(1, 1, 2, 2, 32, 32, 1)  \
+    X(2, 2, 2, 2, 32, 16, 1)  \
+    X(2, 2, 4, 4, 16, 16, 1)  \
+    X(2, 2, 4, 4, 8, 32, 1)   \
+    X(2, 4, 4, 2, 16, 32, 1)  \
+    X(2, 4, 4, 2, 32, 32, 1)  \
+    X(4, 2, 2, 4, 16, 32, 1)  \
+    X(4, 4, 2, 2, 16, 32, 1)  \
+    X(2, 2, 1, 1, 32, 32, 2)  \
+    X(2, 2, 1, 1, 64, 32, 4)  \
+    X(2, 2, 2, 2, 32, 32, 2)  \
+    X(2, 2, 2, 2, 64, 32, 2)
 
 static const MfmaCfg kF32Cfgs[] = {
-#define X(wm, wn, tm, tn, bk, blk) {wm, wn, tm, tn, bk, blk},
+#define X(wm, wn, tm, tn, bk, blk, wk) {wm, wn, tm, tn, bk, blk, wk},
     GAT_F32_MFMA_CFGS(X)
 #undef X
 };
@@ -50,14 +56,14 @@ MfmaCfg mfma_f32_cfg(int i) { return kF32Cfgs[i]; }
 bool mfma_f32_has(const MfmaCfg& c) {
     for (const MfmaCfg& k : kF32Cfgs)
         if (k.waves_m == c.waves_m && k.waves_n == c.waves_n && k.tm == c.tm && k.tn == c.tn &&
-            k.bk == c.bk && k.blk == c.blk)
+            k.bk == c.bk && k.blk == c.blk && k.wk == c.wk)
             return true;
     return false;
 }
 
-template <int WM, int WN, int TM, int TN, int BK, int BLK>
+template <int WM, int WN, int TM, int TN, int BK, int BLK, int WK>
 static int launch_one(const GemmArgs& a, hipStream_t st) {
-    typedef SgemmShape<WM, WN, TM, TN, BK, BLK> S;
+    typedef SgemmShape<WM, WN, TM, TN, BK, BLK, WK> S;
     const int tiles_m = (int)((a.M + S::BM - 1) / S::BM), tiles_n = (int)((a.N + S::BN - 1) / S::BN);
     const float* A = (const float*)a.A;
     const float* B = (const float*)a.B;
@@ -66,14 +72,14 @@ static int launch_one(const GemmArgs& a, hipStream_t st) {
     typedef void (*kfn)(const float*, const float*, float*, int, int, int, int64_t, int64_t, int64_t,
                         int64_t, int64_t, int64_t, float, float, int, int, int, int);
     kfn f;
-    if (!a.transA && !a.transB) f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, false, false>;
-    else if (a.transA && !a.transB) f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, true, false>;
-    else if (!a.transA && a.transB) f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, false, true>;
-    else f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, true, true>;
+    if (!a.transA && !a.transB) f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, WK, false, false>;
+    else if (a.transA && !a.transB) f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, WK, true, false>;
+    else if (!a.transA && a.transB) f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, WK, false, true>;
+    else f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, WK, true, true>;
     if (S::SMEM > 65536) GAT_HIP_TRY(allow_dynamic_lds((const void*)f, S::SMEM));
     const int group_m = tiles_m < 8 ? tiles_m : 8;
     dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)a.batch);
-    hipLaunchKernelGGL(f, grid, dim3(WM * WN * 64), S::SMEM, st, A, B, (float*)a.C, (int)a.M, (int)a.N,
+    hipLaunchKernelGGL(f, grid, dim3(WM * WN * WK * 64), S::SMEM, st, A, B, (float*)a.C, (int)a.M, (int)a.N,
                        (int)a.K, a.lda, a.ldb, a.ldc, a.strideA, a.strideB, a.strideC, (float)a.alpha,
                        (float)a.beta, tiles_m, tiles_n, group_m, vec_ok);
     GAT_HIP_TRY(hipGetLastError());
@@ -83,10 +89,10 @@ static int launch_one(const GemmArgs& a, hipStream_t st) {
 int launch_gemm_mfma_f32(const MfmaCfg& c, const GemmArgs& a, hipStream_t st) {
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return ST_OK;
     if (a.M > INT32_MAX || a.N > INT32_MAX || a.K > INT32_MAX) return ST_INVALID;
-#define X(p_wm, p_wn, p_tm, p_tn, p_bk, p_blk)                                                  \
+#define X(p_wm, p_wn, p_tm, p_tn, p_bk, p_blk, p_wk)                                            \
     if (c.waves_m == p_wm && c.waves_n == p_wn && c.tm == p_tm && c.tn == p_tn && c.bk == p_bk &&  \
-        c.blk == p_blk)                                                                           \
-        return launch_one<p_wm, p_wn, p_tm, p_tn, p_bk, p_blk>(a, st);
+        c.blk == p_blk && c.wk == p_wk)                                                           \
+        return launch_one<p_wm, p_wn, p_tm, p_tn, p_bk, p_blk, p_wk>(a, st);
     GAT_F32_MFMA_CFGS(X)
 #undef X
     set_error("mfma f32: configuration not instantiated");

@@ -230,6 +230,10 @@ static int launch_one64(const GemmArgs& a, hipStream_t st) {
 
 int launch_gemm_mfma_f64(const MfmaCfg& c, const GemmArgs& a, hipStream_t st) {
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return ST_OK;
+    if (c.wk != 1) {
+        set_error("k-split wave groups exist for FP32 tiles only");
+        return ST_UNSUPPORTED;
+    }
     if (a.M > INT32_MAX || a.N > INT32_MAX || a.K > INT32_MAX) return ST_INVALID;
     if (c.blk != 16) {
         set_error("mfma f64: only 16x16 blocks exist");

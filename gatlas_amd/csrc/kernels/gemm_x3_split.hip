@@ -150,7 +150,7 @@ static const MfmaCfg kX3Split{2, 4, 4, 2, 32, 32};
 
 bool x3_split_cfg(const MfmaCfg& c) {
     return c.waves_m == kX3Split.waves_m && c.waves_n == kX3Split.waves_n && c.tm == kX3Split.tm &&
-           c.tn == kX3Split.tn && c.bk == kX3Split.bk && c.blk == kX3Split.blk;
+           c.tn == kX3Split.tn && c.bk == kX3Split.bk && c.blk == kX3Split.blk && c.wk == 1;
 }
 MfmaCfg x3_split_cfg_value() { return kX3Split; }
 

@@ -12,7 +12,12 @@
 //     one is multiplied, then written to the other LDS buffer: one barrier
 //     per K step, and no write-after-read hazard on the LDS tiles;
 //   * epilogue C = alpha*acc + beta*C (C untouched when beta == 0), with
-//     each half-wave storing 32 consecutive floats of a row.
+//     each half-wave storing 32 consecutive floats of a row;
+//   * WK > 1 splits every staged K tile between WK groups of WM x WN waves
+//     (group g takes the g-th run of its 8- or 16-deep steps): a small tile
+//     then keeps WK waves per SIMD busy instead of one.  After the K loop
+//     the groups exchange partial sums through LDS; each adds the WK
+//     partials of its share of the outputs in group order and stores it.
 // MFMA k-lane mapping: MFMA s of a group takes, in lane l, the k index
 // kk + 4*(l/32) + s (32x32x2) or kk + 4*(l/16) + s (16x16x4).  A and B use
 // the same mapping, so the product is the full sum over the group's k.
@@ -118,31 +123,39 @@ struct TileIO {
     }
 };
 
-template <int WM, int WN, int TM, int TN, int BK, int BLK>
+template <int WM, int WN, int TM, int TN, int BK, int BLK, int WK = 1>
 struct SgemmShape {
-    static constexpr int NT = WM * WN * 64;
+    static constexpr int NT = WM * WN * WK * 64;
     static constexpr int BM = WM * TM * BLK, BN = WN * TN * BLK;
     static constexpr int LDK = BK + 4;
-    static constexpr size_t SMEM = 2 * (size_t)(BM + BN) * LDK * sizeof(float);
+    static constexpr size_t STAGES = 2 * (size_t)(BM + BN) * LDK * sizeof(float);
+    // partial sums the WK wave groups exchange after the K loop: each sends
+    // the (WK-1)/WK of its accumulators that other groups finish
+    static constexpr size_t RED = (size_t)(WK - 1) * WM * WN * TM * TN * BLK * BLK * sizeof(float);
+    static constexpr size_t SMEM = STAGES > RED ? STAGES : RED;
 };
 
 // Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C.
-// Must be called by all WM*WN*64 threads of the group; leaves LDS free
+// Must be called by all WM*WN*WK*64 threads of the group; leaves LDS free
 // (ends after a barrier).
-template <int WM, int WN, int TM, int TN, int BK, int BLK, bool TA, bool TB>
+template <int WM, int WN, int TM, int TN, int BK, int BLK, bool TA, bool TB, int WK = 1>
 __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const float* __restrict__ B,
                                            float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb,
                                            int64_t ldc, float alpha, float beta, int m0, int n0, bool vec_ok,
                                            float* smem) {
     typedef MfmaF32<BLK> Op;
-    typedef SgemmShape<WM, WN, TM, TN, BK, BLK> S;
+    typedef SgemmShape<WM, WN, TM, TN, BK, BLK, WK> S;
     constexpr int NT = S::NT, BM = S::BM, BN = S::BN, LDK = S::LDK;
     static_assert(BK % Op::KGROUP == 0, "BK must cover whole MFMA k-groups");
+    constexpr int NSTEP = BK / Op::KGROUP;  // MFMA k-groups per staged K tile
+    static_assert(NSTEP % WK == 0, "the k-split groups must share a stage's steps evenly");
+    constexpr int NS = NSTEP / WK;  // steps per wave and stage
     typedef TileIO<BM, BK, NT, TA> IoA;   // A is direct when op(A) rows are k-contiguous
     typedef TileIO<BN, BK, NT, !TB> IoB;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave / WN, wn = wave % WN;
+    const int wmn = wave % (WM * WN), wk = wave / (WM * WN);  // wk = 0 when WK == 1
+    const int wm = wmn / WN, wn = wmn % WN;
     const bool full_mn = vec_ok && (m0 + BM <= M) && (n0 + BN <= N);
 
     typename Op::acc_t acc[TM][TN];
@@ -169,7 +182,7 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
 
     const int arow = wm * TM * BLK + Op::lane_row(lane);
     const int brow = wn * TN * BLK + Op::lane_row(lane);
-    const int kq = Op::lane_kq(lane);
+    const int kq = Op::lane_kq(lane) + wk * NS * Op::KGROUP;  // this k-group's run of steps
 
     for (int t = 0; t < KT; ++t) {
         const float* As = smem + (t & 1) * (BM + BN) * LDK;
@@ -178,10 +191,9 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
         // The next stage goes to LDS halfway through this one's steps, not
         // after the last: its stores, and the barrier after them, then
         // overlap the MFMAs of the second half still in the pipeline.
-        constexpr int NSTEP = BK / Op::KGROUP;
-        constexpr int STORE_AT = NSTEP / 2 - 1 < 0 ? 0 : NSTEP / 2 - 1;
+        constexpr int STORE_AT = NS / 2 - 1 < 0 ? 0 : NS / 2 - 1;
 #pragma unroll
-        for (int st = 0; st < NSTEP; ++st) {
+        for (int st = 0; st < NS; ++st) {
             const int kk = st * Op::KGROUP;
             f4v a[TM], b[TN];
 #pragma unroll
@@ -203,12 +215,50 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
         __syncthreads();
     }
 
+    // Epilogue.  With WK > 1 every wave group finishes NREG/WK of each
+    // block's accumulator registers: group g owns registers [g*RS, (g+1)*RS)
+    // and receives the other groups' partial sums of them through LDS (the
+    // K loop ended with a barrier, so the stage buffers are free), adding the
+    // WK partials in group order.  The C reads and stores of the epilogue
+    // are thereby spread over all waves.
+    constexpr int RS = Op::NREG / WK;
+    static_assert(Op::NREG % WK == 0, "each wave group must own whole accumulator registers");
+    if constexpr (WK > 1) {
+        // partial sums of group g for owner o != g
+        auto slot = [&](int g, int o, int i, int j, int r) {
+            const int region = g * (WK - 1) + (o < g ? o : o - 1);
+            return ((((region * WM * WN + wmn) * TM + i) * TN + j) * RS + r % RS) * 64 + lane;
+        };
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int r = 0; r < Op::NREG; ++r)
+                    if (r / RS != wk) smem[slot(wk, r / RS, i, j, r)] = acc[i][j][r];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int r = 0; r < Op::NREG; ++r)
+                    if (r / RS == wk) {
+                        float v = wk == 0 ? acc[i][j][r] : smem[slot(0, wk, i, j, r)];
+#pragma unroll
+                        for (int g = 1; g < WK; ++g) v += g == wk ? acc[i][j][r] : smem[slot(g, wk, i, j, r)];
+                        acc[i][j][r] = v;
+                    }
+        __syncthreads();  // LDS free again before the caller's next tile
+    }
+
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < Op::NREG; ++r) {
+                if (WK > 1 && r / RS != wk) continue;
                 const int row = m0 + wm * TM * BLK + i * BLK + Op::out_row(lane, r);
                 const int col = n0 + wn * TN * BLK + j * BLK + Op::out_col(lane);
                 if (row < M && col < N) {

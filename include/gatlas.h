@@ -124,9 +124,13 @@ int gat_gemm_strided(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, 
                      size_t strideB, double beta, void *C, size_t ldc, size_t strideC, size_t batch);
 
 /* MFMA tile configuration: waves_m x waves_n wavefronts per work-group, each
- * owning tm x tn blocks of blk x blk outputs; LDS K depth bk. */
+ * owning tm x tn blocks of blk x blk outputs; LDS K depth bk.  wk > 1
+ * (FP32 only): wk groups of waves_m x waves_n wavefronts share each staged K
+ * tile, each taking a run of its k steps, and their sums are added at the
+ * end; 0 or 1 means no split. */
 typedef struct gat_tile_cfg {
     int waves_m, waves_n, tm, tn, bk, blk;
+    int wk;
 } gat_tile_cfg;
 
 /* gat_gemm with an explicit MFMA tile configuration (used by the tuner). */

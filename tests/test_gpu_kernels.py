@@ -236,10 +236,11 @@ def test_sgemm_mfma_default(ctx, ta, tb, M, N, K):
 
 @pytest.mark.parametrize("cfg", Context.tile_cfgs(_capi.GAT_F32), ids=str)
 def test_sgemm_mfma_every_cfg(ctx, cfg):
+    # ragged edges in the general form, then whole tiles as a pure product
     for ta, tb in [(0, 0), (1, 0), (0, 1), (1, 1)]:
-        for (M, N, K) in [(264, 136, 72), (512, 512, 256)]:
+        for (M, N, K, alpha, beta) in [(264, 136, 72, 1.5, -0.5), (512, 512, 256, 1.0, 0.0)]:
             _say("sgemm cfg", cfg, ta, tb, M, N, K)
-            cw, nw, _ = _gemm_case(ctx, M, N, K, ta, tb, 1.0, 0.0, cfg=cfg)
+            cw, nw, _ = _gemm_case(ctx, M, N, K, ta, tb, alpha, beta, cfg=cfg)
             assert cw <= F32_TOL, (cfg, ta, tb, M, N, K, cw)
 
 
