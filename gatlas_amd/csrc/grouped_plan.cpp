@@ -16,7 +16,8 @@ GroupedPlan::~GroupedPlan() {
 }
 
 MfmaCfg GroupedPlan::shape_cfg(int dtype, int shape) {
-    if (dtype == DT_BF16 || dtype == DT_BF16X3)
+    if (dtype == DT_BF16X3) return shape == 0 ? MfmaCfg{2, 2, 2, 2, 16, 32} : MfmaCfg{2, 2, 1, 1, 32, 32};
+    if (dtype == DT_BF16)
         return shape == 0 ? MfmaCfg{2, 2, 2, 2, 32, 32} : MfmaCfg{2, 2, 1, 1, 64, 32};
     return shape == 0 ? MfmaCfg{2, 2, 2, 2, 32, 32} : MfmaCfg{2, 2, 1, 1, 32, 32};
 }
@@ -151,8 +152,9 @@ int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n,
         GAT_HIP_TRY(hipMemcpyAsync(d_work_, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice,
                                    ctx.stream));
     GAT_HIP_TRY(hipStreamSynchronize(ctx.stream));
-    // two resident work-groups per CU (LDS-limited), never more than the tiles
-    grid_ = std::max(1, std::min<int>(2 * (ctx.num_cu > 0 ? ctx.num_cu : 256), (int)std::max<size_t>(1, nwork_)));
+    // the resident work-groups (LDS-limited per CU), never more than the tiles
+    grid_ = std::max(1, std::min<int>(grouped_wgs_per_cu(form_) * (ctx.num_cu > 0 ? ctx.num_cu : 256),
+                                      (int)std::max<size_t>(1, nwork_)));
     if (use_graph) {
         rc = grouped_gemm_prepare(form_);  // attribute calls stay outside the capture
         if (rc) return rc;

@@ -22,8 +22,11 @@
 // aligned rows stage straight into LDS (bf16_dma_tile.hpp); the others, and
 // every FP32 and split-BF16x3 tile, stage through registers.  The
 // split-BF16x3 form splits FP32 operands into bf16 hi/lo planes while a
-// stage is stored to LDS (bf16_tile.hpp, MODE_X3): the same routine and
-// K depth as the single-problem x3 kernel of that tile, so the bits agree.
+// stage is stored to LDS (bf16_tile.hpp, MODE_X3): the routine of the
+// single-problem x3 kernel of that tile shape, staged 16 (128 x 128) or 32
+// (64 x 64) deep so that three groups fit a CU.  The staging depth does not
+// change an x3 tile's bits (its products run in 16-deep steps in k order
+// at any depth), so a problem's result equals a single launch's.
 #include "bf16_dma_tile.hpp"
 #include "bf16_tile.hpp"
 #include "grouped.hpp"
@@ -71,10 +74,10 @@ __device__ __forceinline__ void x3_tile_shape(const GroupedProblemDev& p, int sh
                                               __bf16* smem) {
     const bool vec = (p.flags & GP_VEC) != 0;
     if (shape == 0)
-        bf16_tile<2, 2, 2, 2, 32, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+        bf16_tile<2, 2, 2, 2, 16, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
                                                    p.beta, m0, n0, vec, smem);
     else
-        bf16_tile<2, 2, 1, 1, 64, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+        bf16_tile<2, 2, 1, 1, 32, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
                                                    p.beta, m0, n0, vec, smem);
 }
 
@@ -83,8 +86,8 @@ __device__ __forceinline__ void x3_tile_shape(const GroupedProblemDev& p, int sh
 // access in it -- starts aligned.
 template <int FORM>
 constexpr size_t grouped_tile_bytes() {
-    typedef Bf16Shape<2, 2, 2, 2, 32, MODE_X3> X0;
-    typedef Bf16Shape<2, 2, 1, 1, 64, MODE_X3> X1;
+    typedef Bf16Shape<2, 2, 2, 2, 16, MODE_X3> X0;
+    typedef Bf16Shape<2, 2, 1, 1, 32, MODE_X3> X1;
     constexpr size_t a = FORM == GP_FORM_F32    ? SgemmShape<2, 2, 2, 2, 32, 32>::SMEM
                          : FORM == GP_FORM_BF16 ? Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM
                                                 : X0::SMEM;
@@ -97,34 +100,23 @@ constexpr size_t grouped_tile_bytes() {
     return ((ab > c ? ab : c) + 15) / 16 * 16;
 }
 
-// Byte offset of the group's next work index in LDS.  FP32 and BF16: a word
-// after the tile buffers.  Split-BF16x3: the two 80 KiB buffers of the 128 x
-// 128 tile fill the 160 KiB of a CU at two groups per CU, so the index sits
-// in the last 4 bytes of the block instead -- k columns BK..BK+7 of the last
-// row of the last plane are row padding that the x3 tile routine never reads
-// or writes (its loads and stores cover k < BK), and the 64 x 64 x3 tile's
-// buffers end before it.
+// Work-groups per CU, as LDS allows (160 KiB per CU, tile buffers plus the
+// 16-byte work-index slot of each group), at most three.
 template <int FORM>
-constexpr size_t grouped_next_offset() {
-    if constexpr (FORM == GP_FORM_X3) {
-        typedef Bf16Shape<2, 2, 2, 2, 32, MODE_X3> X0;
-        typedef Bf16Shape<2, 2, 1, 1, 64, MODE_X3> X1;
-        static_assert(X0::SMEM % 16 == 0 && X0::SMEM >= X1::SMEM + 16, "64x64 x3 buffers must end before the slot");
-        static_assert(2 * X0::SMEM <= 163840, "two 128x128 x3 groups must fit one CU's LDS");
-        return X0::SMEM - 4;  // padding column BK+6..BK+7 (bf16) of the last staged row
-    } else {
-        return grouped_tile_bytes<FORM>();
-    }
+constexpr int grouped_wgs_per_cu() {
+    constexpr int n = (int)(163840 / (grouped_tile_bytes<FORM>() + 16));
+    return n > 3 ? 3 : n;
 }
 
-// Two work-groups per CU (LDS-limited): the register allocator is asked for
-// two waves per SIMD, which the union of all tile routines would exceed.
+// The register allocator is asked for as many waves per SIMD as work-groups
+// fit a CU (a group has one wave per SIMD), which the union of all tile
+// routines would otherwise exceed.
 template <int FORM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void grouped_gemm_kernel(const GroupedProblemDev* __restrict__ probs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grouped_wgs_per_cu<FORM>()))) void grouped_gemm_kernel(const GroupedProblemDev* __restrict__ probs,
                                                            const int4* __restrict__ work, GroupedQueues q,
                                                            int* __restrict__ counter) {
     extern __shared__ __attribute__((aligned(16))) float gsmem[];
-    int& next = *(int*)((char*)gsmem + grouped_next_offset<FORM>());
+    int& next = *(int*)((char*)gsmem + grouped_tile_bytes<FORM>());
     const int home = (int)(blockIdx.x % GP_XCDS);
     int qi = 0;  // queues tried so far: the home queue first, then the others in turn
     for (;;) {
@@ -181,9 +173,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
 
 size_t grouped_smem_bytes(int form) {
     switch (form) {
-        case GP_FORM_F32: return grouped_next_offset<GP_FORM_F32>() + 16;
-        case GP_FORM_BF16: return grouped_next_offset<GP_FORM_BF16>() + 16;
-        default: return grouped_tile_bytes<GP_FORM_X3>();  // the index lives inside the tile block
+        case GP_FORM_F32: return grouped_tile_bytes<GP_FORM_F32>() + 16;
+        case GP_FORM_BF16: return grouped_tile_bytes<GP_FORM_BF16>() + 16;
+        default: return grouped_tile_bytes<GP_FORM_X3>() + 16;
+    }
+}
+
+int grouped_wgs_per_cu(int form) {
+    switch (form) {
+        case GP_FORM_F32: return grouped_wgs_per_cu<GP_FORM_F32>();
+        case GP_FORM_BF16: return grouped_wgs_per_cu<GP_FORM_BF16>();
+        default: return grouped_wgs_per_cu<GP_FORM_X3>();
     }
 }
 

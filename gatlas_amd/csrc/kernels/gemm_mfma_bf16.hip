@@ -61,7 +61,9 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_simd<WM, WN, TM, TN, BK, MODE>
     X(1, 1, 2, 2, 64, 3)  \
     X(2, 4, 4, 2, 32, 1)  \
     X(2, 4, 4, 2, 64, 1)  \
-    X(2, 4, 4, 2, 16, 2)
+    X(2, 4, 4, 2, 16, 2)  \
+    X(2, 2, 2, 2, 16, 2)  \
+    X(2, 2, 1, 1, 32, 2)
 
 struct Bf16CfgEntry {
     MfmaCfg cfg;

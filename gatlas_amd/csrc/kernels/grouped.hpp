@@ -38,6 +38,8 @@ enum : int {
 };
 
 size_t grouped_smem_bytes(int form);
+// Resident work-groups per CU of a form's persistent kernel (LDS-limited).
+int grouped_wgs_per_cu(int form);
 // Sets the kernel's dynamic LDS limit (once); needed before graph capture.
 int grouped_gemm_prepare(int form);
 // `work` holds int4 entries {problem, m0, n0, 0} in per-XCD queues `q`;

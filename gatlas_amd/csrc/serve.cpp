@@ -314,7 +314,10 @@ int Server::launch_batch(int kind, std::vector<int>& pend, int slot, Launch& L) 
         std::memcpy(b.h_stage + woff, work.data(), work.size() * sizeof(int4));
         GAT_HIP_TRY(hipMemcpyAsync(b.d_stage, b.h_stage, woff + work.size() * sizeof(int4), hipMemcpyHostToDevice,
                                    st));
-        const int grid = (int)std::max<size_t>(1, std::min<size_t>(2 * (size_t)std::max(ctx_->num_cu, 1), work.size()));
+        const int grid = (int)std::max<size_t>(
+            1, std::min<size_t>((size_t)grouped_wgs_per_cu(GroupedPlan::form_of(cfg_.gemm_dtype)) *
+                                    (size_t)std::max(ctx_->num_cu, 1),
+                                work.size()));
         rc = launch_grouped_gemm(GroupedPlan::form_of(cfg_.gemm_dtype), (const GroupedProblemDev*)b.d_stage, (const int4*)(b.d_stage + woff), q,
                                  b.d_counter, grid, st);
         if (rc) return rc;

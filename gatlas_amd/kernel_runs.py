@@ -6,6 +6,7 @@ work:
 
 Workloads (inputs U[-1,1] from the device generator):
   sgemm8192    C = 1.5*A*B - 0.5*C, FP32, M=N=K=8192, no transpose
+  sgemm1024    C = A*B, FP32, M=N=K=1024, no transpose (default tile rule)
   dgemm8192    the same in FP64
   bf16x3_8192  split-BF16x3 (FP32 in/out), C = A*B
   bf16_8192    BF16 inputs, FP32 out, C = A*B
@@ -14,6 +15,7 @@ Workloads (inputs U[-1,1] from the device generator):
   saxpy        z = 1.5*x + y, FP32, n = 2^28
   grouped512   512 FP32 problems (M, N, K log-uniform in [64, 4096]) in one
                persistent grouped launch (untuned tile rule)
+  grouped512x3 the same problems in split-BF16x3
 """
 import argparse
 import sys
@@ -33,7 +35,12 @@ def _sq(ctx, n, dt=np.float32, count=3):
 
 def run(ctx, name, reps):
     n = 8192
-    if name in ("sgemm8192", "dgemm8192"):
+    if name == "sgemm1024":
+        m = 1024
+        A, B, C = _sq(ctx, m)
+        for _ in range(reps):
+            ctx.gemm(A, B, C, m, m, m)
+    elif name in ("sgemm8192", "dgemm8192"):
         A, B, C = _sq(ctx, n, np.float32 if name[0] == "s" else np.float64)
         for _ in range(reps):
             ctx.gemm(A, B, C, n, n, n, alpha=1.5, beta=-0.5)
@@ -65,7 +72,7 @@ def run(ctx, name, reps):
             ctx.fill_uniform(a, 5, i)
         for _ in range(reps):
             ctx.gemv(A, x, y, m, m, transA=name.endswith("t"), alpha=1.0, beta=1.0)
-    elif name == "grouped512":
+    elif name in ("grouped512", "grouped512x3"):
         # 512 problems, M, N, K log-uniform in [64, 4096] (multiples of 16),
         # random transposes, FP32; one persistent launch per repetition
         rng = np.random.default_rng(0x67525550)
@@ -79,7 +86,7 @@ def run(ctx, name, reps):
             ctx.fill_uniform(dA, 21, 2 * i)
             ctx.fill_uniform(dB, 21, 2 * i + 1)
             probs.append(dict(transA=int(tr[i, 0]), transB=int(tr[i, 1]), M=M, N=N, K=K, A=dA, B=dB, C=dC))
-        g = ctx.grouped(probs, _capi.GAT_F32, use_graph=False)
+        g = ctx.grouped(probs, _capi.GAT_BF16X3 if name.endswith("x3") else _capi.GAT_F32, use_graph=False)
         for _ in range(reps):
             g.run()
         ctx.sync()

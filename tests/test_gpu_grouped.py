@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 SHAPE_CFG = {_capi.GAT_F32: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 32, 32)],
              _capi.GAT_BF16: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 64, 32)],
-             _capi.GAT_BF16X3: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 64, 32)]}
+             _capi.GAT_BF16X3: [(2, 2, 2, 2, 16, 32), (2, 2, 1, 1, 32, 32)]}
 FORMS = [_capi.GAT_F32, _capi.GAT_BF16, _capi.GAT_BF16X3]
 FORM_IDS = ["f32", "bf16", "bf16x3"]
 
