@@ -39,6 +39,7 @@ __global__ __launch_bounds__(WM* WN * WK * 64) void sgemm_mfma_kernel(
     X(2, 4, 4, 2, 32, 32, 1)  \
     X(4, 2, 2, 4, 16, 32, 1)  \
     X(4, 4, 2, 2, 16, 32, 1)  \
+    X(4, 4, 2, 2, 32, 32, 1)  \
     X(2, 2, 1, 1, 32, 32, 2)  \
     X(2, 2, 1, 1, 64, 32, 4)  \
     X(2, 2, 2, 2, 32, 32, 2)  \
