@@ -251,11 +251,13 @@ def sweep_bench(ctx, sizes=(1024, 2048, 4096, 8192)):
         for i, a in enumerate((A, B, C)):
             ctx.fill_uniform(a, 0x6A7A5EED, i)
         for n in sizes:
-            steps = max(2, min(20, int(4e12 / (2.0 * n ** 3))))
+            # about 10 ms of timed work per case (at least 3 launches), after
+            # 3 warm-up launches: short runs of small sizes read the clock ramp
+            steps = max(3, min(400, int(1.2e12 / (2.0 * n ** 3))))
             for ta in (0, 1):
                 for tb in (0, 1):
                     ms, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n, bool(ta), bool(tb), 1.5, -0.5, dtype=dt),
-                                  steps, 1)
+                                  steps, 3)
                     out["%s_%d_%s%s" % (name, n, "T" if ta else "N", "T" if tb else "N")] = round(
                         2.0 * n ** 3 / (ms * 1e-3) / 1e12, 2)
         for a in (A, B, C):

@@ -6,6 +6,11 @@
 // MFMAs.  MFMA s of a k-group takes, in lane l, k = kk + 2*(l/16) + s.
 // Output block layout (measured on gfx950; differs from the FP32 16x16x4
 // form): lane l, accumulator r holds row 4*r + l/16 of column l%16.
+// WK > 1 splits each staged K tile between WK groups of WM x WN waves, as
+// the FP32 k-split tiles do: group g takes the g-th run of the stage's
+// 8-deep steps, the groups exchange partial sums through LDS after the K
+// loop, and group g adds the WK partials of accumulator register g*4/WK..
+// in group order and stores those outputs.
 #include "../kernels.hpp"
 
 namespace gat {
@@ -80,21 +85,35 @@ struct TileIO64 {
     }
 };
 
-template <int WM, int WN, int TM, int TN, int BK, bool TA, bool TB>
-__global__ __launch_bounds__(WM* WN * 64) void dgemm_mfma_kernel(
+template <int WM, int WN, int TM, int TN, int BK, int WK>
+struct DgemmShape {
+    static constexpr int BM = WM * TM * 16, BN = WN * TN * 16;
+    static constexpr size_t STAGES = 2 * (size_t)(BM + BN) * (BK + 2) * sizeof(double);
+    // partial sums the WK wave groups exchange after the K loop
+    static constexpr size_t RED = (size_t)(WK - 1) * WM * WN * TM * TN * 256 * sizeof(double);
+    static constexpr size_t SMEM = STAGES > RED ? STAGES : RED;
+};
+
+template <int WM, int WN, int TM, int TN, int BK, int WK, bool TA, bool TB>
+__global__ __launch_bounds__(WM* WN * WK * 64) void dgemm_mfma_kernel(
     const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C, int M, int N, int K,
     int64_t lda, int64_t ldb, int64_t ldc, int64_t sA, int64_t sB, int64_t sC, double alpha, double beta,
     int tiles_m, int tiles_n, int group_m, int vec_ok) {
-    constexpr int NT = WM * WN * 64;
+    constexpr int NT = WM * WN * WK * 64;
     constexpr int BM = WM * TM * 16, BN = WN * TN * 16;
     constexpr int LDK = BK + 2;
     static_assert(BK % 8 == 0, "BK must cover whole k-groups of 8");
+    constexpr int NS = BK / 8 / WK;  // 8-deep steps per wave and stage
+    static_assert(NS * 8 * WK == BK, "the k-split groups must share a stage's steps evenly");
+    constexpr int RS = 4 / WK;  // accumulator registers each group finishes
+    static_assert(RS * WK == 4, "each wave group must own whole accumulator registers");
     typedef TileIO64<BM, BK, NT, TA> IoA;
     typedef TileIO64<BN, BK, NT, !TB> IoB;
 
     extern __shared__ double smemd[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave / WN, wn = wave % WN;
+    const int wmn = wave % (WM * WN), wk = wave / (WM * WN);  // wk = 0 when WK == 1
+    const int wm = wmn / WN, wn = wmn % WN;
 
     A += (int64_t)blockIdx.y * sA;
     B += (int64_t)blockIdx.y * sB;
@@ -130,14 +149,14 @@ __global__ __launch_bounds__(WM* WN * 64) void dgemm_mfma_kernel(
 
     const int arow = wm * TM * 16 + (lane & 15);
     const int brow = wn * TN * 16 + (lane & 15);
-    const int kq = 2 * (lane >> 4);
+    const int kq = 2 * (lane >> 4) + wk * NS * 8;  // this k-group's run of steps
 
     for (int t = 0; t < KT; ++t) {
         const double* As = smemd + (t & 1) * (BM + BN) * LDK;
         const double* Bs = As + BM * LDK;
         if (t + 1 < KT) gload(t + 1);
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 8) {
+        for (int kk = 0; kk < NS * 8; kk += 8) {
             d2v a[TM], b[TN];
 #pragma unroll
             for (int i = 0; i < TM; ++i) a[i] = *(const d2v*)(As + (arow + i * 16) * LDK + kk + kq);
@@ -159,12 +178,41 @@ __global__ __launch_bounds__(WM* WN * 64) void dgemm_mfma_kernel(
         __syncthreads();
     }
 
+    if constexpr (WK > 1) {
+        // partial sums of group g for owner o != g (the stage buffers are free)
+        auto slot = [&](int g, int o, int i, int j, int r) {
+            const int region = g * (WK - 1) + (o < g ? o : o - 1);
+            return ((((region * WM * WN + wmn) * TM + i) * TN + j) * RS + r % RS) * 64 + lane;
+        };
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (r / RS != wk) smemd[slot(wk, r / RS, i, j, r)] = acc[i][j][r];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (r / RS == wk) {
+                        double v = wk == 0 ? acc[i][j][r] : smemd[slot(0, wk, i, j, r)];
+#pragma unroll
+                        for (int g = 1; g < WK; ++g) v += g == wk ? acc[i][j][r] : smemd[slot(g, wk, i, j, r)];
+                        acc[i][j][r] = v;
+                    }
+    }
+
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+                if (WK > 1 && r / RS != wk) continue;
                 const int row = m0 + wm * TM * 16 + i * 16 + r * 4 + (lane >> 4);
                 const int col = n0 + wn * TN * 16 + j * 16 + (lane & 15);
                 if (row < M && col < N) {
@@ -176,17 +224,21 @@ __global__ __launch_bounds__(WM* WN * 64) void dgemm_mfma_kernel(
             }
 }
 
-// Instantiated configurations: (WM, WN, TM, TN, BK).
+// Instantiated configurations: (WM, WN, TM, TN, BK, WK); the WK > 1 entries
+// are for problems whose tiles give fewer than two work-groups per CU.
 #define GAT_F64_MFMA_CFGS(X) \
-    X(2, 2, 2, 2, 16)        \
-    X(2, 2, 4, 4, 16)        \
-    X(2, 2, 4, 2, 16)        \
-    X(2, 2, 2, 2, 32)        \
-    X(2, 2, 4, 4, 8)         \
-    X(1, 1, 2, 2, 32)
+    X(2, 2, 2, 2, 16, 1)     \
+    X(2, 2, 4, 4, 16, 1)     \
+    X(2, 2, 4, 2, 16, 1)     \
+    X(2, 2, 2, 2, 32, 1)     \
+    X(2, 2, 4, 4, 8, 1)      \
+    X(1, 1, 2, 2, 32, 1)     \
+    X(2, 2, 2, 2, 16, 2)     \
+    X(2, 2, 2, 2, 32, 2)     \
+    X(2, 2, 2, 2, 32, 4)
 
 static const MfmaCfg kF64Cfgs[] = {
-#define X(wm, wn, tm, tn, bk) {wm, wn, tm, tn, bk, 16},
+#define X(wm, wn, tm, tn, bk, wk) {wm, wn, tm, tn, bk, 16, wk},
     GAT_F64_MFMA_CFGS(X)
 #undef X
 };
@@ -197,15 +249,15 @@ MfmaCfg mfma_f64_cfg(int i) { return kF64Cfgs[i]; }
 bool mfma_f64_has(const MfmaCfg& c) {
     for (const MfmaCfg& k : kF64Cfgs)
         if (k.waves_m == c.waves_m && k.waves_n == c.waves_n && k.tm == c.tm && k.tn == c.tn &&
-            k.bk == c.bk && k.blk == c.blk)
+            k.bk == c.bk && k.blk == c.blk && k.wk == c.wk)
             return true;
     return false;
 }
 
-template <int WM, int WN, int TM, int TN, int BK>
+template <int WM, int WN, int TM, int TN, int BK, int WK>
 static int launch_one64(const GemmArgs& a, hipStream_t st) {
     constexpr int BM = WM * TM * 16, BN = WN * TN * 16;
-    constexpr size_t smem = 2 * (size_t)(BM + BN) * (BK + 2) * sizeof(double);
+    constexpr size_t smem = DgemmShape<WM, WN, TM, TN, BK, WK>::SMEM;
     const int tiles_m = (int)((a.M + BM - 1) / BM), tiles_n = (int)((a.N + BN - 1) / BN);
     const double* A = (const double*)a.A;
     const double* B = (const double*)a.B;
@@ -214,14 +266,14 @@ static int launch_one64(const GemmArgs& a, hipStream_t st) {
     typedef void (*kfn)(const double*, const double*, double*, int, int, int, int64_t, int64_t, int64_t,
                         int64_t, int64_t, int64_t, double, double, int, int, int, int);
     kfn f;
-    if (!a.transA && !a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, false, false>;
-    else if (a.transA && !a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, true, false>;
-    else if (!a.transA && a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, false, true>;
-    else f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, true, true>;
+    if (!a.transA && !a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, WK, false, false>;
+    else if (a.transA && !a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, WK, true, false>;
+    else if (!a.transA && a.transB) f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, WK, false, true>;
+    else f = dgemm_mfma_kernel<WM, WN, TM, TN, BK, WK, true, true>;
     if (smem > 65536) GAT_HIP_TRY(allow_dynamic_lds((const void*)f, smem));
     const int group_m = tiles_m < 8 ? tiles_m : 8;
     dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)a.batch);
-    hipLaunchKernelGGL(f, grid, dim3(WM * WN * 64), smem, st, A, B, (double*)a.C, (int)a.M, (int)a.N,
+    hipLaunchKernelGGL(f, grid, dim3(WM * WN * WK * 64), smem, st, A, B, (double*)a.C, (int)a.M, (int)a.N,
                        (int)a.K, a.lda, a.ldb, a.ldc, a.strideA, a.strideB, a.strideC, a.alpha, a.beta,
                        tiles_m, tiles_n, group_m, vec_ok);
     GAT_HIP_TRY(hipGetLastError());
@@ -230,18 +282,15 @@ static int launch_one64(const GemmArgs& a, hipStream_t st) {
 
 int launch_gemm_mfma_f64(const MfmaCfg& c, const GemmArgs& a, hipStream_t st) {
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return ST_OK;
-    if (c.wk != 1) {
-        set_error("k-split wave groups exist for FP32 tiles only");
-        return ST_UNSUPPORTED;
-    }
     if (a.M > INT32_MAX || a.N > INT32_MAX || a.K > INT32_MAX) return ST_INVALID;
     if (c.blk != 16) {
         set_error("mfma f64: only 16x16 blocks exist");
         return ST_UNSUPPORTED;
     }
-#define X(p_wm, p_wn, p_tm, p_tn, p_bk)                                                              \
-    if (c.waves_m == p_wm && c.waves_n == p_wn && c.tm == p_tm && c.tn == p_tn && c.bk == p_bk)      \
-        return launch_one64<p_wm, p_wn, p_tm, p_tn, p_bk>(a, st);
+#define X(p_wm, p_wn, p_tm, p_tn, p_bk, p_wk)                                                        \
+    if (c.waves_m == p_wm && c.waves_n == p_wn && c.tm == p_tm && c.tn == p_tn && c.bk == p_bk &&    \
+        c.wk == p_wk)                                                                                 \
+        return launch_one64<p_wm, p_wn, p_tm, p_tn, p_bk, p_wk>(a, st);
     GAT_F64_MFMA_CFGS(X)
 #undef X
     set_error("mfma f64: configuration not instantiated");

@@ -289,10 +289,12 @@ def test_dgemm_mfma(ctx, ta, tb, M, N, K):
 
 def test_dgemm_every_cfg(ctx):
     for cfg in ctx.tile_cfgs(_capi.GAT_F64):
-        for ta, tb in [(0, 0), (1, 1)]:
+        for ta, tb in [(0, 0), (1, 0), (0, 1), (1, 1)]:
             _say("dgemm cfg", cfg, ta, tb)
             cw, _, _ = _gemm_case(ctx, 160, 96, 64, ta, tb, 1.0, 0.0, dt=np.float64, cfg=cfg)
             assert cw <= F64_TOL, (cfg, cw)
+            cw, _, _ = _gemm_case(ctx, 200, 136, 96, ta, tb, 1.5, -0.5, dt=np.float64, cfg=cfg)
+            assert cw <= F64_TOL, (cfg, "general", cw)
 
 
 def test_sgemm_8192_general_form_sampled(ctx):
