@@ -11,7 +11,8 @@
 //     c ^ (r & 7): the 32 rows x 2 chunks of one fragment read cover every
 //     bank group evenly.
 //   row-contiguous operand (A transposed, B not transposed): [k][row] as in
-//     memory, chunk c of k-row k stored at c ^ (4 * (k & 1)); fragments are
+//     memory, chunk c of k-row k stored at c ^ (4 * (k & 3)) (mod chunks
+//     per k-row); fragments are
 //     read with ds_read_b64_tr_b16.  Within each 16-lane group, lane i's
 //     half s is element (i % 4) of the 8 bytes addressed by lane 4s + i/4,
 //     so lane j addressing (k-row kb + j/4, row n0 + 4*(j % 4)) hands lane
@@ -44,7 +45,13 @@ struct DmaOperand {
     static_assert(KCONTIG || (sizeof(T) == 2 && CPR >= 8 && CPR <= 64 && 64 % CPR == 0),
                   "row-contiguous tile: 2-byte elements, 64..512 rows");
 
-    __device__ static __forceinline__ int tswz(int k) { return 4 * (k & 1); }
+    // A k-row of the row-contiguous layout is ROWS*2 bytes, a multiple of
+    // the 256 bytes of one pass over the 64 banks, so every k-row starts at
+    // bank 0.  A fragment read touches 64 contiguous bytes of each of k-rows
+    // kb..kb+3 and kb+8..kb+11; XOR-ing their chunks by 4*(k mod 4) puts the
+    // four k-rows of each 16-lane group on four different 64-byte bank
+    // windows, so the 512 bytes of one read take two passes, not four.
+    __device__ static __forceinline__ int tswz(int k) { return (4 * (k & 3)) & (CPR - 1); }
 
     // Wavefront w fetches its blocks of the stage starting at k0 into `lds`.
     __device__ static __forceinline__ void issue(const T* src, int64_t ld, int r0, int k0, char* lds, int w,
@@ -103,7 +110,7 @@ struct DmaOperand {
             const int n = rbase + 16 * (g & 1) + 4 * (j & 3);
             const int k = kk + 8 * (g >> 1) + (j >> 2);
             const int off0 = k * (ROWS * 2) + 16 * ((n >> 3) ^ tswz(k)) + 2 * (n & 7);
-            const int off1 = off0 + 4 * (ROWS * 2);  // k + 4: same parity, same swizzle
+            const int off1 = off0 + 4 * (ROWS * 2);  // k + 4: same k mod 4, same swizzle
             const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4_t*)(lds + off0));
             const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4_t*)(lds + off1));
             const s16x4 v[2] = {lo, hi};
