@@ -203,10 +203,9 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
         small = MfmaCfg{2, 2, 1, 1, 64, 32};
         tiny = MfmaCfg{1, 1, 2, 2, 64, 32};
     } else {
-        // 256 x 256, 16 waves; 32-deep stages, or 16-deep when A is stored
-        // transposed (its transposing LDS stores then cost more than the
-        // halved barriers save: TT 8192^3 127 vs 133 TFLOP/s)
-        big = MfmaCfg{4, 4, 2, 2, a.transA ? 16 : 32, 32};
+        // 256 x 256, 16 waves, 32-deep stages: interior tiles stage straight
+        // into LDS (sgemm_dma_tile.hpp)
+        big = MfmaCfg{4, 4, 2, 2, 32, 32};
         mid = MfmaCfg{2, 2, 2, 2, 32, 32};
         small = MfmaCfg{2, 2, 1, 1, 32, 32};
         tiny = MfmaCfg{1, 1, 2, 2, 32, 32};

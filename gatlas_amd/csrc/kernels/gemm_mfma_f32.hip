@@ -1,9 +1,26 @@
 // FP32 GEMM on gfx950 matrix cores: one work-group per macro tile, walked
 // in XCD-aware supertile order; the tile computation is sgemm_tile
-// (sgemm_tile.hpp), shared with the persistent grouped kernel.
+// (sgemm_tile.hpp), shared with the persistent grouped kernel.  The 256 x
+// 256 tile staged 32 deep runs its interior tiles (K a multiple of 32,
+// 16-byte aligned rows) on the direct-to-LDS routine (sgemm_dma_tile.hpp),
+// which gives the same bits.
+#include "sgemm_dma_tile.hpp"
 #include "sgemm_tile.hpp"
 
 namespace gat {
+
+// Configurations whose interior tiles take the direct-to-LDS routine.
+template <int WM, int WN, int TM, int TN, int BK, int BLK, int WK>
+constexpr bool sgemm_has_dma() {
+    return WK == 1 && BK == 32 && BLK == 32 && WM * TM * 32 == 256 && WN * TN * 32 == 256;
+}
+
+template <int WM, int WN, int TM, int TN, int BK, int BLK, int WK>
+constexpr size_t sgemm_smem() {
+    constexpr size_t reg = SgemmShape<WM, WN, TM, TN, BK, BLK, WK>::SMEM;
+    constexpr size_t dma = sgemm_has_dma<WM, WN, TM, TN, BK, BLK, WK>() ? SgemmDmaShape<WM, WN, TM, TN>::SMEM : 0;
+    return reg > dma ? reg : dma;
+}
 
 template <int WM, int WN, int TM, int TN, int BK, int BLK, int WK, bool TA, bool TB>
 __global__ __launch_bounds__(WM* WN * WK * 64) void sgemm_mfma_kernel(
@@ -17,6 +34,13 @@ __global__ __launch_bounds__(WM* WN * WK * 64) void sgemm_mfma_kernel(
     C += (int64_t)blockIdx.y * sC;
     int tmi, tni;
     mfma_tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, group_m, tmi, tni);
+    if constexpr (sgemm_has_dma<WM, WN, TM, TN, BK, BLK, WK>()) {
+        const int m0 = tmi * S::BM, n0 = tni * S::BN;
+        if (vec_ok && K >= 32 && K % 32 == 0 && m0 + S::BM <= M && n0 + S::BN <= N) {
+            sgemm_tile_dma<WM, WN, TM, TN, TA, TB>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, m0, n0, smem);
+            return;
+        }
+    }
     sgemm_tile<WM, WN, TM, TN, BK, BLK, TA, TB, WK>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, tmi * S::BM,
                                                     tni * S::BN, vec_ok != 0, smem);
 }
@@ -77,10 +101,11 @@ static int launch_one(const GemmArgs& a, hipStream_t st) {
     else if (a.transA && !a.transB) f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, WK, true, false>;
     else if (!a.transA && a.transB) f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, WK, false, true>;
     else f = sgemm_mfma_kernel<WM, WN, TM, TN, BK, BLK, WK, true, true>;
-    if (S::SMEM > 65536) GAT_HIP_TRY(allow_dynamic_lds((const void*)f, S::SMEM));
+    constexpr size_t smem = sgemm_smem<WM, WN, TM, TN, BK, BLK, WK>();
+    if (smem > 65536) GAT_HIP_TRY(allow_dynamic_lds((const void*)f, smem));
     const int group_m = tiles_m < 8 ? tiles_m : 8;
     dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)a.batch);
-    hipLaunchKernelGGL(f, grid, dim3(WM * WN * WK * 64), S::SMEM, st, A, B, (float*)a.C, (int)a.M, (int)a.N,
+    hipLaunchKernelGGL(f, grid, dim3(WM * WN * WK * 64), smem, st, A, B, (float*)a.C, (int)a.M, (int)a.N,
                        (int)a.K, a.lda, a.ldb, a.ldc, a.strideA, a.strideB, a.strideC, (float)a.alpha,
                        (float)a.beta, tiles_m, tiles_n, group_m, vec_ok);
     GAT_HIP_TRY(hipGetLastError());

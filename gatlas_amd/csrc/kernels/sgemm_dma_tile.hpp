@@ -1,0 +1,245 @@
+// Direct-to-LDS variant of the FP32 matrix-core macro tile (BK = 32) for
+// interior tiles: both operands go from HBM into LDS with gfx950's 16-byte
+// global_load_lds, so no register holds a staged tile, no LDS store
+// instruction runs, and the next K stage lands in LDS while the MFMAs run
+// on the current one.  The products are the register-staged tile's
+// (sgemm_tile.hpp, 32x32x2 blocks): lane l takes k = kk + 4*(l/32) + s in
+// MFMA s of an 8-deep step, in the same order, so the two paths agree bit
+// for bit.
+//
+// Layouts of one 32-deep stage in LDS:
+//   k-contiguous operand (op(A) rows when A is not transposed, op(B)^T rows
+//     when B is transposed): [row][k], 128 bytes (32 floats) per row,
+//     16-byte chunk c of row r at c ^ ((r >> 1) & 7); a fragment (four k
+//     values of one row) is one 16-byte read.  Rows r and r + 1 fall in the
+//     two halves of a 256-byte pass over the banks and rows r, r + 2, ...
+//     differ in the swizzle, so the 16 rows one read covers per 16 lanes
+//     hit 16 different 16-byte bank slots (the c ^ (r & 7) swizzle of the
+//     BF16 layout leaves rows r and r + 8 on one slot).  B's rows keep
+//     c ^ (r & 7) when A is transposed, which measured faster there.
+//   row-contiguous operand (A transposed, B not transposed): [k][row] as in
+//     memory, ROWS floats per k-row; 16-byte chunk c of k-row k sits at
+//     c ^ (8 * ((k >> 2) & 1)).  A fragment is four 4-byte reads, k-rows
+//     kk + s (lanes 0..31) and kk + 4 + s (lanes 32..63); k-rows are a
+//     multiple of 256 bytes long, so without the swizzle the two halves of
+//     the wavefront would read the same 32 banks.
+#pragma once
+
+#include "bf16_dma_tile.hpp"
+#include "sgemm_tile.hpp"
+
+namespace gat {
+
+template <int ROWS, int NW>
+struct DmaRowsF32 {
+    static constexpr int CPR = ROWS / 4;            // 16-byte chunks per k-row
+    static constexpr int BYTES = 32 * ROWS * 4;     // one 32-deep stage
+    static constexpr int BLOCKS = BYTES / 1024;     // one wavefront instruction each
+    static constexpr int PER_WAVE = BLOCKS / NW;
+    static constexpr int KPB = 64 / CPR;            // k-rows per block
+    static_assert(PER_WAVE * NW == BLOCKS, "blocks must split evenly over the wavefronts");
+    static_assert(CPR >= 16 && CPR <= 64 && 64 % CPR == 0, "row-contiguous FP32 tile: 64..256 rows");
+
+    __device__ static __forceinline__ int rsw(int k) { return 8 * ((k >> 2) & 1); }
+
+    // Wavefront w fetches its blocks of the stage starting at k0 into `lds`.
+    __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
+                                                 int lane) {
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int b = i * NW + w;
+            const int k = b * KPB + lane / CPR;
+            const float* g = src + (int64_t)(k0 + k) * ld + r0 + 4 * ((lane % CPR) ^ rsw(k));
+            __builtin_amdgcn_global_load_lds((glb_void_t*)g, (lds_void_t*)(lds + b * 1024), 16, 0, 0);
+        }
+    }
+
+    // B-side operand of MFMA s of the 8-deep step at kk for row rbase + lane%32:
+    // element (k = kk + 4*(lane/32) + s, row)
+    __device__ static __forceinline__ void frag(const char* lds, int rbase, int kk, int lane, float (&v)[4]) {
+        const int n = rbase + (lane & 31);
+        const int k = kk + 4 * (lane >> 5);  // kk is a multiple of 8: k .. k+3 share the swizzle
+        const float* p = (const float*)(lds + k * (ROWS * 4) + 16 * ((n >> 2) ^ rsw(k))) + (n & 3);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) v[s] = p[s * ROWS];
+    }
+};
+
+// k-contiguous FP32 operand of a 32-deep K stage: ROWS rows of 128 bytes.
+template <int ROWS, int NW, int SW>
+struct DmaColsF32 {
+    static constexpr int BYTES = ROWS * 128;
+    static constexpr int BLOCKS = BYTES / 1024;  // one wavefront instruction each (8 rows)
+    static constexpr int PER_WAVE = BLOCKS / NW;
+    static_assert(PER_WAVE * NW == BLOCKS, "blocks must split evenly over the wavefronts");
+
+    __device__ static __forceinline__ int kswz(int r) { return SW ? (r >> 1) & 7 : r & 7; }
+
+    __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
+                                                 int lane) {
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int b = i * NW + w;
+            const int row = 8 * b + (lane >> 3);
+            const float* g = src + (int64_t)(r0 + row) * ld + k0 + 4 * ((lane & 7) ^ kswz(row));
+            __builtin_amdgcn_global_load_lds((glb_void_t*)g, (lds_void_t*)(lds + b * 1024), 16, 0, 0);
+        }
+    }
+
+    // operand of MFMA s of the 8-deep step at kk for row rbase + lane%32:
+    // element (row, k = kk + 4*(lane/32) + s)
+    __device__ static __forceinline__ void frag(const char* lds, int rbase, int kk, int lane, float (&v)[4]) {
+        const int row = rbase + (lane & 31);
+        const int c = (kk >> 2) + (lane >> 5);
+        const f4v f = *(const f4v*)(lds + row * 128 + 16 * (c ^ kswz(row)));
+#pragma unroll
+        for (int s = 0; s < 4; ++s) v[s] = f[s];
+    }
+};
+
+template <int ROWS, bool KCONTIG, int NW, int SW = 1>
+struct DmaF32Operand;
+
+template <int ROWS, int NW, int SW>
+struct DmaF32Operand<ROWS, true, NW, SW> {
+    typedef DmaColsF32<ROWS, NW, SW> Op;
+    static constexpr int BYTES = Op::BYTES;
+    __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
+                                                 int lane) {
+        Op::issue(src, ld, r0, k0, lds, w, lane);
+    }
+    __device__ static __forceinline__ void frag(const char* lds, int rbase, int kk, int lane, float (&v)[4]) {
+        Op::frag(lds, rbase, kk, lane, v);
+    }
+};
+
+template <int ROWS, int NW, int SW>
+struct DmaF32Operand<ROWS, false, NW, SW> {
+    typedef DmaRowsF32<ROWS, NW> Op;
+    static constexpr int BYTES = Op::BYTES;
+    __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
+                                                 int lane) {
+        Op::issue(src, ld, r0, k0, lds, w, lane);
+    }
+    __device__ static __forceinline__ void frag(const char* lds, int rbase, int kk, int lane, float (&v)[4]) {
+        Op::frag(lds, rbase, kk, lane, v);
+    }
+};
+
+template <int WM, int WN, int TM, int TN>
+struct SgemmDmaShape {
+    static constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    static constexpr size_t SMEM = 2 * (size_t)(BM + BN) * 32 * sizeof(float);
+};
+
+// Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C for
+// a tile wholly inside M x N, K a positive multiple of 32, 16-byte aligned
+// operand rows; all WM*WN*64 threads of the group call it; ends after a
+// barrier.
+template <int WM, int WN, int TM, int TN, bool TA, bool TB>
+__device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, const float* __restrict__ B,
+                                               float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb,
+                                               int64_t ldc, float alpha, float beta, int m0, int n0, float* smem) {
+    constexpr int NW = WM * WN;
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    // k-contiguous rows swizzled by (r >> 1) & 7, except B's when A is
+    // transposed: measured at 8192^3, TT runs 144 TFLOP/s with r & 7 there
+    // against 138 with (r >> 1) & 7, while NT gains 141 -> 143 and NN
+    // (A's swizzle) 137.5 -> 139.6 from (r >> 1) & 7.
+    typedef DmaF32Operand<BM, !TA, NW, 1> OpA;       // op(A) rows are k-contiguous unless A is transposed
+    typedef DmaF32Operand<BN, TB, NW, TA ? 0 : 1> OpB;  // op(B)^T rows are k-contiguous when B is transposed
+    constexpr int STAGE = OpA::BYTES + OpB::BYTES;
+    char* const sb = (char*)smem;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = w / WN, wn = w % WN;
+
+    f16v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int KT = K / 32;
+    float a0[TM][4], b0[TN][4], a1[TM][4], b1[TN][4];
+    auto frags = [&](const char* stage, int kk, float(&a)[TM][4], float(&b)[TN][4]) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) OpA::frag(stage, wm * TM * 32 + i * 32, kk, lane, a[i]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) OpB::frag(stage + OpA::BYTES, wn * TN * 32 + j * 32, kk, lane, b[j]);
+    };
+    auto mma = [&](const float(&a)[TM][4], const float(&b)[TN][4]) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+    };
+
+    // one MFMA, then one LDS read and one global load if any remain, ...
+    // (8192^3 NN 139.4 -> 142.9, TN 144.5 -> 146.7 TFLOP/s against the
+    // compiler's own order)
+    auto interleave = [&]() {
+#pragma unroll
+        for (int i = 0; i < TM * TN * 4; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load (to LDS)
+        }
+    };
+
+    OpA::issue(A, lda, m0, 0, sb, w, lane);
+    OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    frags(sb, 0, a0, b0);
+
+    // Software pipeline over the four 8-deep steps of a stage: the
+    // fragments of step s+1 are read while the MFMAs of step s run; the wait
+    // for the next stage and the barrier come before the last step, whose
+    // MFMAs run under the reads of the next stage's first fragments.  The
+    // next stage is fetched unconditionally (the last iteration fetches its
+    // own stage again into the idle buffer).
+    for (int t = 0; t < KT; ++t) {
+        const char* cur = sb + (t & 1) * STAGE;
+        char* nxt = sb + ((t + 1) & 1) * STAGE;
+        const int kn = (t + 1 < KT ? t + 1 : t) * 32;
+        OpA::issue(A, lda, m0, kn, nxt, w, lane);
+        OpB::issue(B, ldb, n0, kn, nxt + OpA::BYTES, w, lane);
+        frags(cur, 8, a1, b1);
+        mma(a0, b0);
+        interleave();
+        frags(cur, 16, a0, b0);
+        mma(a1, b1);
+        interleave();
+        frags(cur, 24, a1, b1);
+        mma(a0, b0);
+        interleave();
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
+        __syncthreads();
+        frags(nxt, 0, a0, b0);
+        mma(a1, b1);
+        interleave();
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * TM * 32 + i * 32 + mfma32_row(lane, r);
+                const int col = n0 + wn * TN * 32 + j * 32 + mfma32_col(lane);
+                float* p = C + (int64_t)row * ldc + col;
+                float v = alpha * acc[i][j][r];
+                if (beta != 0.0f) v += beta * gload<float>(p);
+                gstore(p, v);
+            }
+    __syncthreads();  // LDS free for the caller's next tile
+}
+
+}  // namespace gat

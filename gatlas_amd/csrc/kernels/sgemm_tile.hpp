@@ -192,6 +192,10 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
         // after the last: its stores, and the barrier after them, then
         // overlap the MFMAs of the second half still in the pipeline.
         constexpr int STORE_AT = NS / 2 - 1 < 0 ? 0 : NS / 2 - 1;
+        // With B staged through transposing stores (B not transposed), let
+        // the scheduler interleave the fragment reads with the MFMAs: NN at
+        // 8192^3 135 -> 137.5 TFLOP/s; NT loses 0.6 % with it, so not there.
+        if constexpr (!TB) __builtin_amdgcn_iglp_opt(0);
 #pragma unroll
         for (int st = 0; st < NS; ++st) {
             const int kk = st * Op::KGROUP;
