@@ -18,9 +18,10 @@
 //
 // Tile shapes: 0 = 128 x 128 (2 x 2 waves of 2 x 2 32x32 blocks),
 //              1 = 64 x 64 (2 x 2 waves of one 32x32 block).
-// BF16 interior tiles of problems with K >= 64 a multiple of 8 and 16-byte
-// aligned rows stage straight into LDS (bf16_dma_tile.hpp); the others, and
-// every FP32 and split-BF16x3 tile, stage through registers.  The
+// BF16 interior tiles of problems with K >= 64 a multiple of 8, and FP32
+// interior tiles of problems with K a multiple of 32, with 16-byte aligned
+// rows, stage straight into LDS (bf16_dma_tile.hpp, sgemm_dma_tile.hpp);
+// the others, and every split-BF16x3 tile, stage through registers.  The
 // split-BF16x3 form splits FP32 operands into bf16 hi/lo planes while a
 // stage is stored to LDS (bf16_tile.hpp, MODE_X3): the routine of the
 // single-problem x3 kernel of that tile shape, staged 16 (128 x 128) or 32
@@ -30,19 +31,34 @@
 #include "bf16_dma_tile.hpp"
 #include "bf16_tile.hpp"
 #include "grouped.hpp"
+#include "sgemm_dma_tile.hpp"
 #include "sgemm_tile.hpp"
 
 namespace gat {
 
+// Interior FP32 tiles of problems with K a multiple of 32 and 16-byte
+// aligned rows stage straight into LDS (sgemm_dma_tile.hpp, the same bits).
 template <bool TA, bool TB>
 __device__ __forceinline__ void f32_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0, float* smem) {
     const bool vec = (p.flags & GP_VEC) != 0;
-    if (shape == 0)
-        sgemm_tile<2, 2, 2, 2, 32, 32, TA, TB>((const float*)p.A, (const float*)p.B, p.C, p.M, p.N, p.K, p.lda,
-                                               p.ldb, p.ldc, p.alpha, p.beta, m0, n0, vec, smem);
-    else
-        sgemm_tile<2, 2, 1, 1, 32, 32, TA, TB>((const float*)p.A, (const float*)p.B, p.C, p.M, p.N, p.K, p.lda,
-                                               p.ldb, p.ldc, p.alpha, p.beta, m0, n0, vec, smem);
+    const bool dma = vec && p.K >= 32 && p.K % 32 == 0;
+    const float* A = (const float*)p.A;
+    const float* B = (const float*)p.B;
+    if (shape == 0) {
+        if (dma && m0 + 128 <= p.M && n0 + 128 <= p.N)
+            sgemm_tile_dma<2, 2, 2, 2, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0,
+                                               n0, smem);
+        else
+            sgemm_tile<2, 2, 2, 2, 32, 32, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
+                                                   m0, n0, vec, smem);
+    } else {
+        if (dma && m0 + 64 <= p.M && n0 + 64 <= p.N)
+            sgemm_tile_dma<2, 2, 1, 1, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0,
+                                               n0, smem);
+        else
+            sgemm_tile<2, 2, 1, 1, 32, 32, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
+                                                   m0, n0, vec, smem);
+    }
 }
 
 template <bool TA, bool TB>
@@ -95,7 +111,9 @@ constexpr size_t grouped_tile_bytes() {
                          : FORM == GP_FORM_BF16 ? Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM
                                                 : X1::SMEM;
     // direct-to-LDS stages of shape 0
-    constexpr size_t c = FORM == GP_FORM_BF16 ? Bf16DmaShape<2, 2, 2, 2>::SMEM : 0;
+    constexpr size_t c = FORM == GP_FORM_BF16 ? Bf16DmaShape<2, 2, 2, 2>::SMEM
+                         : FORM == GP_FORM_F32 ? SgemmDmaShape<2, 2, 2, 2>::SMEM
+                                               : 0;
     constexpr size_t ab = a > b ? a : b;
     return ((ab > c ? ab : c) + 15) / 16 * 16;
 }
