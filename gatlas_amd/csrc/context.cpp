@@ -168,8 +168,8 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
 // Untuned tile rule: the largest tile of the form that still gives at least
 // one work-group per CU of an MI355X (256 CUs), counting batched problems;
 // a problem too small to fill the chip with any tile gets the 64 x 64 tile
-// of four waves (one wave when a side is below 256).  FP32 tiles that give
-// fewer than two work-groups per CU take their k-split form (two or four
+// of four waves (one wave when a side is below 256).  FP32 64 x 64 tiles
+// that give fewer than two work-groups per CU take their k-split form (two
 // wave groups share each K stage), which keeps more than one wave per SIMD.
 // At 1024^3, 2048^3 and 4096^3 FP32 this picks the same tiles as an
 // exhaustive timing of the table.
@@ -203,18 +203,22 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
         small = MfmaCfg{2, 2, 1, 1, 64, 32};
         tiny = MfmaCfg{1, 1, 2, 2, 64, 32};
     } else {
-        // 256 x 256, 16 waves, 32-deep stages: interior tiles stage straight
-        // into LDS (sgemm_dma_tile.hpp)
-        big = MfmaCfg{4, 4, 2, 2, 32, 32};
+        // 256 x 256, 32-deep stages: interior tiles stage straight into LDS
+        // (sgemm_dma_tile.hpp).  NN takes 8 waves of 4 x 2 blocks, the other
+        // transpose pairs 16 waves of 2 x 2 (8192^3 NN 144.3 vs 142.6, TN
+        // 146.9 vs 143.4 TFLOP/s on one box)
+        big = (!a.transA && !a.transB) ? MfmaCfg{2, 4, 4, 2, 32, 32} : MfmaCfg{4, 4, 2, 2, 32, 32};
         mid = MfmaCfg{2, 2, 2, 2, 32, 32};
         small = MfmaCfg{2, 2, 1, 1, 32, 32};
         tiny = MfmaCfg{1, 1, 2, 2, 32, 32};
     }
     const bool split = dtype == DT_F32;
     if (fills(big)) return big;
-    if (fills(mid)) return split && tiles(mid) < 2 * kFillTiles ? MfmaCfg{2, 2, 2, 2, 32, 32, 2} : mid;
+    // the 128 x 128 tile stages interior tiles straight into LDS in its
+    // unsplit form, which the tuner prefers at 2048^3 (124 vs 116 TFLOP/s)
+    if (fills(mid)) return mid;
     if (fills(small) || mn >= 256)
-        return split && tiles(small) < 2 * kFillTiles ? MfmaCfg{2, 2, 1, 1, 64, 32, 4} : small;
+        return split && tiles(small) < 2 * kFillTiles ? MfmaCfg{2, 2, 1, 1, 32, 32, 2} : small;
     return tiny;
 }
 

@@ -1,9 +1,9 @@
 // FP32 GEMM on gfx950 matrix cores: one work-group per macro tile, walked
 // in XCD-aware supertile order; the tile computation is sgemm_tile
-// (sgemm_tile.hpp), shared with the persistent grouped kernel.  The 256 x
-// 256 tile staged 32 deep runs its interior tiles (K a multiple of 32,
-// 16-byte aligned rows) on the direct-to-LDS routine (sgemm_dma_tile.hpp),
-// which gives the same bits.
+// (sgemm_tile.hpp), shared with the persistent grouped kernel.  The tiles
+// staged 32 deep (32x32 blocks, no k-split or a two-group one) run their
+// interior tiles (K a multiple of 32, 16-byte aligned rows) on the
+// direct-to-LDS routine (sgemm_dma_tile.hpp), which gives the same bits.
 #include "sgemm_dma_tile.hpp"
 #include "sgemm_tile.hpp"
 
@@ -12,13 +12,16 @@ namespace gat {
 // Configurations whose interior tiles take the direct-to-LDS routine.
 template <int WM, int WN, int TM, int TN, int BK, int BLK, int WK>
 constexpr bool sgemm_has_dma() {
-    return WK == 1 && BK == 32 && BLK == 32 && WM * TM * 32 == 256 && WN * TN * 32 == 256;
+    constexpr int NW = WM * WN * WK, BM = WM * TM * 32, BN = WN * TN * 32;
+    return (WK == 1 || WK == 2) && BK == 32 && BLK == 32 && BM >= 64 && BN >= 64 && BM <= 256 && BN <= 256 &&
+           (BM / 8) % NW == 0 && (BN / 8) % NW == 0;
 }
 
 template <int WM, int WN, int TM, int TN, int BK, int BLK, int WK>
 constexpr size_t sgemm_smem() {
     constexpr size_t reg = SgemmShape<WM, WN, TM, TN, BK, BLK, WK>::SMEM;
-    constexpr size_t dma = sgemm_has_dma<WM, WN, TM, TN, BK, BLK, WK>() ? SgemmDmaShape<WM, WN, TM, TN>::SMEM : 0;
+    constexpr size_t dma =
+        sgemm_has_dma<WM, WN, TM, TN, BK, BLK, WK>() ? SgemmDmaShape<WM, WN, TM, TN, WK>::SMEM : 0;
     return reg > dma ? reg : dma;
 }
 
@@ -37,7 +40,7 @@ __global__ __launch_bounds__(WM* WN * WK * 64) void sgemm_mfma_kernel(
     if constexpr (sgemm_has_dma<WM, WN, TM, TN, BK, BLK, WK>()) {
         const int m0 = tmi * S::BM, n0 = tni * S::BN;
         if (vec_ok && K >= 32 && K % 32 == 0 && m0 + S::BM <= M && n0 + S::BN <= N) {
-            sgemm_tile_dma<WM, WN, TM, TN, TA, TB>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, m0, n0, smem);
+            sgemm_tile_dma<WM, WN, TM, TN, TA, TB, WK>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, m0, n0, smem);
             return;
         }
     }

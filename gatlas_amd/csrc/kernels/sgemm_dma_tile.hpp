@@ -126,33 +126,41 @@ struct DmaF32Operand<ROWS, false, NW, SW> {
     }
 };
 
-template <int WM, int WN, int TM, int TN>
+template <int WM, int WN, int TM, int TN, int WK = 1>
 struct SgemmDmaShape {
     static constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
-    static constexpr size_t SMEM = 2 * (size_t)(BM + BN) * 32 * sizeof(float);
+    static constexpr size_t STAGES = 2 * (size_t)(BM + BN) * 32 * sizeof(float);
+    static constexpr size_t RED = SgemmShape<WM, WN, TM, TN, 32, 32, WK>::RED;  // k-split exchange
+    static constexpr size_t SMEM = STAGES > RED ? STAGES : RED;
 };
 
 // Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C for
 // a tile wholly inside M x N, K a positive multiple of 32, 16-byte aligned
-// operand rows; all WM*WN*64 threads of the group call it; ends after a
-// barrier.
-template <int WM, int WN, int TM, int TN, bool TA, bool TB>
+// operand rows; all WM*WN*WK*64 threads of the group call it; ends after a
+// barrier.  WK > 1 splits each stage's four 8-deep steps between WK groups
+// of WM x WN waves as the register-staged k-split tile does (group g takes
+// steps g*4/WK ..), with the same partial-sum exchange, so the bits agree.
+template <int WM, int WN, int TM, int TN, bool TA, bool TB, int WK = 1>
 __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, const float* __restrict__ B,
                                                float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb,
                                                int64_t ldc, float alpha, float beta, int m0, int n0, float* smem) {
-    constexpr int NW = WM * WN;
+    constexpr int NW = WM * WN * WK;  // wavefronts issuing the stage loads
     constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    constexpr int NS = 4 / WK;  // 8-deep steps per wave and stage
+    static_assert(NS == 4 || NS == 2, "k-split tiles split a 32-deep stage into two or four runs");
     // k-contiguous rows swizzled by (r >> 1) & 7, except B's when A is
     // transposed: measured at 8192^3, TT runs 144 TFLOP/s with r & 7 there
     // against 138 with (r >> 1) & 7, while NT gains 141 -> 143 and NN
     // (A's swizzle) 137.5 -> 139.6 from (r >> 1) & 7.
-    typedef DmaF32Operand<BM, !TA, NW, 1> OpA;       // op(A) rows are k-contiguous unless A is transposed
+    typedef DmaF32Operand<BM, !TA, NW, 1> OpA;          // op(A) rows are k-contiguous unless A is transposed
     typedef DmaF32Operand<BN, TB, NW, TA ? 0 : 1> OpB;  // op(B)^T rows are k-contiguous when B is transposed
     constexpr int STAGE = OpA::BYTES + OpB::BYTES;
     char* const sb = (char*)smem;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = w / WN, wn = w % WN;
+    const int wmn = WK == 1 ? w : w % (WM * WN), wk = WK == 1 ? 0 : w / (WM * WN);
+    const int wm = wmn / WN, wn = wmn % WN;
+    const int kb = WK == 1 ? 0 : wk * NS * 8;  // this k-group's run of steps in a stage
 
     f16v acc[TM][TN];
 #pragma unroll
@@ -179,7 +187,6 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
                 for (int j = 0; j < TN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
     };
-
     // one MFMA, then one LDS read and one global load if any remain, ...
     // (8192^3 NN 139.4 -> 142.9, TN 144.5 -> 146.7 TFLOP/s against the
     // compiler's own order)
@@ -196,9 +203,9 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
     OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    frags(sb, 0, a0, b0);
+    frags(sb, kb, a0, b0);
 
-    // Software pipeline over the four 8-deep steps of a stage: the
+    // Software pipeline over the wave's NS 8-deep steps of a stage: the
     // fragments of step s+1 are read while the MFMAs of step s run; the wait
     // for the next stage and the barrier come before the last step, whose
     // MFMAs run under the reads of the next stage's first fragments.  The
@@ -210,36 +217,34 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
         const int kn = (t + 1 < KT ? t + 1 : t) * 32;
         OpA::issue(A, lda, m0, kn, nxt, w, lane);
         OpB::issue(B, ldb, n0, kn, nxt + OpA::BYTES, w, lane);
-        frags(cur, 8, a1, b1);
-        mma(a0, b0);
-        interleave();
-        frags(cur, 16, a0, b0);
-        mma(a1, b1);
-        interleave();
-        frags(cur, 24, a1, b1);
-        mma(a0, b0);
-        interleave();
+        if constexpr (NS == 4) {
+            frags(cur, kb + 8, a1, b1);
+            mma(a0, b0);
+            interleave();
+            frags(cur, kb + 16, a0, b0);
+            mma(a1, b1);
+            interleave();
+            frags(cur, kb + 24, a1, b1);
+            mma(a0, b0);
+            interleave();
+        } else {
+            frags(cur, kb + 8, a1, b1);
+            mma(a0, b0);
+            interleave();
+        }
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
         __syncthreads();
-        frags(nxt, 0, a0, b0);
+        frags(nxt, kb, a0, b0);
         mma(a1, b1);
         interleave();
     }
+    // the k-split exchange reuses the stage buffers: every wave's last
+    // fragment reads must be done
+    if constexpr (WK > 1) __syncthreads();
 
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = m0 + wm * TM * 32 + i * 32 + mfma32_row(lane, r);
-                const int col = n0 + wn * TN * 32 + j * 32 + mfma32_col(lane);
-                float* p = C + (int64_t)row * ldc + col;
-                float v = alpha * acc[i][j][r];
-                if (beta != 0.0f) v += beta * gload<float>(p);
-                gstore(p, v);
-            }
-    __syncthreads();  // LDS free for the caller's next tile
+    sgemm_finish<WM, WN, TM, TN, 32, WK, false>(acc, C, M, N, ldc, alpha, beta, m0, n0, wm, wn, wmn, wk, lane,
+                                                smem);
+    if constexpr (WK == 1) __syncthreads();  // LDS free for the caller's next tile
 }
 
 }  // namespace gat

@@ -135,6 +135,69 @@ struct SgemmShape {
     static constexpr size_t SMEM = STAGES > RED ? STAGES : RED;
 };
 
+// Epilogue of an FP32 macro tile: C = alpha*acc + beta*C for the wave's
+// TM x TN blocks (C untouched when beta == 0; outputs outside M x N are
+// skipped).  With WK > 1 every wave group finishes NREG/WK of each block's
+// accumulator registers: group g owns registers [g*RS, (g+1)*RS) and
+// receives the other groups' partial sums of them through LDS (the caller's
+// K loop ended with a barrier, so the stage buffers are free), adding the
+// WK partials in group order.  The C reads and stores are thereby spread
+// over all waves.  Ends with LDS free for the caller's next tile.  CHECKED
+// false: the tile lies wholly inside M x N, no output is tested.
+template <int WM, int WN, int TM, int TN, int BLK, int WK, bool CHECKED = true>
+__device__ __forceinline__ void sgemm_finish(typename MfmaF32<BLK>::acc_t (&acc)[TM][TN], float* __restrict__ C,
+                                             int M, int N, int64_t ldc, float alpha, float beta, int m0, int n0,
+                                             int wm, int wn, int wmn, int wk, int lane, float* smem) {
+    typedef MfmaF32<BLK> Op;
+    constexpr int RS = Op::NREG / WK;
+    static_assert(Op::NREG % WK == 0, "each wave group must own whole accumulator registers");
+    if constexpr (WK > 1) {
+        // partial sums of group g for owner o != g
+        auto slot = [&](int g, int o, int i, int j, int r) {
+            const int region = g * (WK - 1) + (o < g ? o : o - 1);
+            return ((((region * WM * WN + wmn) * TM + i) * TN + j) * RS + r % RS) * 64 + lane;
+        };
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int r = 0; r < Op::NREG; ++r)
+                    if (r / RS != wk) smem[slot(wk, r / RS, i, j, r)] = acc[i][j][r];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int r = 0; r < Op::NREG; ++r)
+                    if (r / RS == wk) {
+                        float v = wk == 0 ? acc[i][j][r] : smem[slot(0, wk, i, j, r)];
+#pragma unroll
+                        for (int g = 1; g < WK; ++g) v += g == wk ? acc[i][j][r] : smem[slot(g, wk, i, j, r)];
+                        acc[i][j][r] = v;
+                    }
+        __syncthreads();  // LDS free again before the caller's next tile
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < Op::NREG; ++r) {
+                if (WK > 1 && r / RS != wk) continue;
+                const int row = m0 + wm * TM * BLK + i * BLK + Op::out_row(lane, r);
+                const int col = n0 + wn * TN * BLK + j * BLK + Op::out_col(lane);
+                if (!CHECKED || (row < M && col < N)) {
+                    float* p = C + (int64_t)row * ldc + col;
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.0f) v += beta * gload<float>(p);
+                    gstore(p, v);
+                }
+            }
+}
+
 // Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C.
 // Must be called by all WM*WN*WK*64 threads of the group; leaves LDS free
 // (ends after a barrier).
@@ -219,59 +282,7 @@ __device__ __forceinline__ void sgemm_tile(const float* __restrict__ A, const fl
         __syncthreads();
     }
 
-    // Epilogue.  With WK > 1 every wave group finishes NREG/WK of each
-    // block's accumulator registers: group g owns registers [g*RS, (g+1)*RS)
-    // and receives the other groups' partial sums of them through LDS (the
-    // K loop ended with a barrier, so the stage buffers are free), adding the
-    // WK partials in group order.  The C reads and stores of the epilogue
-    // are thereby spread over all waves.
-    constexpr int RS = Op::NREG / WK;
-    static_assert(Op::NREG % WK == 0, "each wave group must own whole accumulator registers");
-    if constexpr (WK > 1) {
-        // partial sums of group g for owner o != g
-        auto slot = [&](int g, int o, int i, int j, int r) {
-            const int region = g * (WK - 1) + (o < g ? o : o - 1);
-            return ((((region * WM * WN + wmn) * TM + i) * TN + j) * RS + r % RS) * 64 + lane;
-        };
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-#pragma unroll
-                for (int r = 0; r < Op::NREG; ++r)
-                    if (r / RS != wk) smem[slot(wk, r / RS, i, j, r)] = acc[i][j][r];
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-#pragma unroll
-                for (int r = 0; r < Op::NREG; ++r)
-                    if (r / RS == wk) {
-                        float v = wk == 0 ? acc[i][j][r] : smem[slot(0, wk, i, j, r)];
-#pragma unroll
-                        for (int g = 1; g < WK; ++g) v += g == wk ? acc[i][j][r] : smem[slot(g, wk, i, j, r)];
-                        acc[i][j][r] = v;
-                    }
-        __syncthreads();  // LDS free again before the caller's next tile
-    }
-
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int r = 0; r < Op::NREG; ++r) {
-                if (WK > 1 && r / RS != wk) continue;
-                const int row = m0 + wm * TM * BLK + i * BLK + Op::out_row(lane, r);
-                const int col = n0 + wn * TN * BLK + j * BLK + Op::out_col(lane);
-                if (row < M && col < N) {
-                    float* p = C + (int64_t)row * ldc + col;
-                    float v = alpha * acc[i][j][r];
-                    if (beta != 0.0f) v += beta * gload<float>(p);
-                    gstore(p, v);
-                }
-            }
+    sgemm_finish<WM, WN, TM, TN, BLK, WK>(acc, C, M, N, ldc, alpha, beta, m0, n0, wm, wn, wmn, wk, lane, smem);
 }
 
 }  // namespace gat

@@ -244,19 +244,22 @@ def test_sgemm_mfma_every_cfg(ctx, cfg):
             assert cw <= F32_TOL, (cfg, ta, tb, M, N, K, cw)
 
 
+DMA_CFGS = [c for c in Context.tile_cfgs(_capi.GAT_F32) if c[4] == 32 and c[5] == 32 and c[6] in (1, 2)]
+
+
+@pytest.mark.parametrize("cfg", DMA_CFGS, ids=str)
 @pytest.mark.parametrize("K", [96, 100])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-def test_sgemm_direct_lds_path(ctx, ta, tb, K):
-    # The 256 x 256 tile staged 32 deep stages interior tiles straight from
-    # HBM into LDS (k-contiguous operands XOR-swizzled, row-contiguous ones
-    # as [k][row] with a half-bank shift every four k-rows) and edge tiles
-    # through registers.  M, N give whole and partial tiles; K = 100 is not a
-    # multiple of 32, so every tile takes the register path.  Checked against
-    # the FP64 oracle, and bit for bit against the register path on the same
-    # data (leading dimensions one element longer, not 16-byte multiples,
-    # force that path on every tile).
+def test_sgemm_direct_lds_path(ctx, ta, tb, K, cfg):
+    # Tiles staged 32 deep (no k-split, or two k-split groups) stage interior
+    # tiles straight from HBM into LDS (k-contiguous operands XOR-swizzled,
+    # row-contiguous ones as [k][row] with a half-bank shift every four
+    # k-rows) and edge tiles through registers.  M, N give whole and partial
+    # 256-wide tiles; K = 100 is not a multiple of 32, so every tile takes the
+    # register path.  Checked against the FP64 oracle, and bit for bit
+    # against the register path on the same data (leading dimensions one
+    # element longer, not 16-byte multiples, force that path on every tile).
     M, N = 512, 384
-    cfg = (4, 4, 2, 2, 32, 32, 1)
     A = uniform_host(M * K, 36, 0)
     B = uniform_host(K * N, 36, 1)
     C0 = uniform_host(M * N, 36, 2)
@@ -266,16 +269,16 @@ def test_sgemm_direct_lds_path(ctx, ta, tb, K):
     dA, dB = ctx.to_device(A), ctx.to_device(B)
     pA, pB = ctx.to_device(pad(A, ra, ca)), ctx.to_device(pad(B, rb, cb))
     C64, MAG = ora.gemm64(A, B, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, C0=C0)
-    _say("sgemm direct-LDS", ta, tb, K)
+    _say("sgemm direct-LDS", cfg, ta, tb, K)
     d1, d2 = ctx.to_device(C0), ctx.to_device(C0)
     ctx.gemm(dA, dB, d1, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, cfg=cfg)
     ctx.gemm(pA, pB, d2, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, lda=ca + 1, ldb=cb + 1, cfg=cfg)
     got, ref = d1.download(), d2.download()
     cw, _, _ = ora.error_metrics(got, C64, MAG)
-    assert cw <= F32_TOL, (ta, tb, K, cw)
+    assert cw <= F32_TOL, (cfg, ta, tb, K, cw)
     cw2, _, _ = ora.error_metrics(ref, C64, MAG)
-    assert cw2 <= F32_TOL, (ta, tb, K, cw2)
-    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (ta, tb, K)
+    assert cw2 <= F32_TOL, (cfg, ta, tb, K, cw2)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (cfg, ta, tb, K)
 
 
 def test_sgemm_test_pattern(ctx):
