@@ -85,6 +85,63 @@ struct TileIO64 {
     }
 };
 
+// Epilogue of an FP64 macro tile: C = alpha*acc + beta*C for the wave's
+// TM x TN 16x16 blocks.  With WK > 1 group g owns accumulator registers
+// [g*RS, (g+1)*RS) of each block and adds the WK partial sums of them, sent
+// through LDS (the stage buffers, free after the caller's final barrier),
+// in group order.  CHECKED false: the tile lies wholly inside M x N.
+template <int WM, int WN, int TM, int TN, int WK, bool CHECKED>
+__device__ __forceinline__ void dgemm_finish(d4v (&acc)[TM][TN], double* __restrict__ C, int M, int N, int64_t ldc,
+                                             double alpha, double beta, int m0, int n0, int wm, int wn, int wmn,
+                                             int wk, int lane, double* smemd) {
+    constexpr int RS = 4 / WK;  // accumulator registers each group finishes
+    static_assert(RS * WK == 4, "each wave group must own whole accumulator registers");
+    if constexpr (WK > 1) {
+        // partial sums of group g for owner o != g (the stage buffers are free)
+        auto slot = [&](int g, int o, int i, int j, int r) {
+            const int region = g * (WK - 1) + (o < g ? o : o - 1);
+            return ((((region * WM * WN + wmn) * TM + i) * TN + j) * RS + r % RS) * 64 + lane;
+        };
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (r / RS != wk) smemd[slot(wk, r / RS, i, j, r)] = acc[i][j][r];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (r / RS == wk) {
+                        double v = wk == 0 ? acc[i][j][r] : smemd[slot(0, wk, i, j, r)];
+#pragma unroll
+                        for (int g = 1; g < WK; ++g) v += g == wk ? acc[i][j][r] : smemd[slot(g, wk, i, j, r)];
+                        acc[i][j][r] = v;
+                    }
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (WK > 1 && r / RS != wk) continue;
+                const int row = m0 + wm * TM * 16 + i * 16 + r * 4 + (lane >> 4);
+                const int col = n0 + wn * TN * 16 + j * 16 + (lane & 15);
+                if (!CHECKED || (row < M && col < N)) {
+                    double* p = C + (int64_t)row * ldc + col;
+                    double v = alpha * acc[i][j][r];
+                    if (beta != 0.0) v += beta * *p;
+                    *p = v;
+                }
+            }
+}
+
 template <int WM, int WN, int TM, int TN, int BK, int WK>
 struct DgemmShape {
     static constexpr int BM = WM * TM * 16, BN = WN * TN * 16;
@@ -93,6 +150,181 @@ struct DgemmShape {
     static constexpr size_t RED = (size_t)(WK - 1) * WM * WN * TM * TN * 256 * sizeof(double);
     static constexpr size_t SMEM = STAGES > RED ? STAGES : RED;
 };
+
+// ---- direct-to-LDS staging (BK = 16, interior tiles) ----
+// One 16-deep stage is 128 bytes of k per operand row.  A global_load_lds of
+// a wavefront writes 1 KiB contiguous in LDS, lane l's 16 bytes at
+// base + 16*l.
+//   k-contiguous operand: [row][k], chunk c (two doubles) of row r at
+//     c ^ ((r >> 1) & 7): the 16 rows a 16-lane group reads at one chunk
+//     hit 16 different 16-byte bank slots.  A fragment is one 16-byte read.
+//   row-contiguous operand: [k][row] as in memory; chunk c of k-row k at
+//     c ^ (8 * ((k >> 1) & 1)).  A fragment is two 8-byte reads, k-rows
+//     kk + 2*(lane/16) + s; the shift puts the k-rows of lane groups 0/2 and
+//     1/3 in different halves of a 256-byte bank pass.
+typedef __attribute__((address_space(3))) void lds_void64_t;
+typedef __attribute__((address_space(1))) void glb_void64_t;
+
+template <int ROWS, int NW>
+struct DmaColsF64 {
+    static constexpr int BYTES = ROWS * 128;
+    static constexpr int BLOCKS = BYTES / 1024;  // one wavefront instruction each (8 rows)
+    static constexpr int PER_WAVE = BLOCKS / NW;
+    static_assert(PER_WAVE >= 1 && PER_WAVE * NW == BLOCKS, "blocks must split evenly over the wavefronts");
+    __device__ static __forceinline__ int kswz(int r) { return (r >> 1) & 7; }
+    __device__ static __forceinline__ void issue(const double* src, int64_t ld, int r0, int k0, char* lds, int w,
+                                                 int lane) {
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int b = i * NW + w;
+            const int row = 8 * b + (lane >> 3);
+            const double* g = src + (int64_t)(r0 + row) * ld + k0 + 2 * ((lane & 7) ^ kswz(row));
+            __builtin_amdgcn_global_load_lds((glb_void64_t*)g, (lds_void64_t*)(lds + b * 1024), 16, 0, 0);
+        }
+    }
+    // operands of the two MFMAs of the 8-deep step at kk, row rbase + lane%16:
+    // k = kk + 2*(lane/16) + s
+    __device__ static __forceinline__ d2v frag(const char* lds, int rbase, int kk, int lane) {
+        const int row = rbase + (lane & 15);
+        const int c = (kk >> 1) + (lane >> 4);
+        return *(const d2v*)(lds + row * 128 + 16 * (c ^ kswz(row)));
+    }
+};
+
+template <int ROWS, int NW>
+struct DmaRowsF64 {
+    static constexpr int CPR = ROWS / 2;         // 16-byte chunks per k-row
+    static constexpr int BYTES = 16 * ROWS * 8;  // one 16-deep stage
+    static constexpr int BLOCKS = BYTES / 1024;
+    static constexpr int PER_WAVE = BLOCKS / NW;
+    static constexpr int KPB = 64 / CPR;  // k-rows per block
+    static_assert(PER_WAVE >= 1 && PER_WAVE * NW == BLOCKS, "blocks must split evenly over the wavefronts");
+    static_assert(CPR >= 16 && CPR <= 64 && 64 % CPR == 0, "row-contiguous FP64 tile: 32..128 rows");
+    __device__ static __forceinline__ int rsw(int k) { return 8 * ((k >> 1) & 1); }
+    __device__ static __forceinline__ void issue(const double* src, int64_t ld, int r0, int k0, char* lds, int w,
+                                                 int lane) {
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int b = i * NW + w;
+            const int k = b * KPB + lane / CPR;
+            const double* g = src + (int64_t)(k0 + k) * ld + r0 + 2 * ((lane % CPR) ^ rsw(k));
+            __builtin_amdgcn_global_load_lds((glb_void64_t*)g, (lds_void64_t*)(lds + b * 1024), 16, 0, 0);
+        }
+    }
+    __device__ static __forceinline__ d2v frag(const char* lds, int rbase, int kk, int lane) {
+        const int n = rbase + (lane & 15);
+        const int k = kk + 2 * (lane >> 4);  // even: k and k + 1 share the shift
+        const double* p = (const double*)(lds + k * (ROWS * 8) + 16 * ((n >> 1) ^ rsw(k))) + (n & 1);
+        return d2v{p[0], p[ROWS]};
+    }
+};
+
+template <int ROWS, bool KCONTIG, int NW>
+struct DmaF64Operand;
+template <int ROWS, int NW>
+struct DmaF64Operand<ROWS, true, NW> : DmaColsF64<ROWS, NW> {};
+template <int ROWS, int NW>
+struct DmaF64Operand<ROWS, false, NW> : DmaRowsF64<ROWS, NW> {};
+
+// Configurations whose interior tiles take the direct-to-LDS routine
+template <int WM, int WN, int TM, int TN, int BK, int WK>
+constexpr bool dgemm_has_dma() {
+    constexpr int NW = WM * WN * WK, BM = WM * TM * 16, BN = WN * TN * 16;
+    return BK == 16 && (WK == 1 || WK == 2) && BM >= 32 && BN >= 32 && BM <= 128 && BN <= 128 &&
+           (BM / 8) % NW == 0 && (BN / 8) % NW == 0 && (BM / 8) >= NW && (BN / 8) >= NW;
+}
+
+// The tile at (m0, n0), wholly inside M x N, K a positive multiple of 16,
+// 16-byte aligned operand rows, staged straight into LDS; the MFMA order is
+// the register path's (k = kk + 2*(lane/16) + s per step, steps in k order,
+// group g of a k-split tile taking step g of each stage), so the bits agree.
+template <int WM, int WN, int TM, int TN, int WK, bool TA, bool TB>
+__device__ __forceinline__ void dgemm_tile_dma(const double* __restrict__ A, const double* __restrict__ B,
+                                               double* __restrict__ C, int M, int N, int K, int64_t lda,
+                                               int64_t ldb, int64_t ldc, double alpha, double beta, int m0, int n0,
+                                               double* smemd) {
+    constexpr int NW = WM * WN * WK;
+    constexpr int BM = WM * TM * 16, BN = WN * TN * 16;
+    constexpr int NS = 2 / WK;  // 8-deep steps per wave and stage
+    typedef DmaF64Operand<BM, !TA, NW> OpA;
+    typedef DmaF64Operand<BN, TB, NW> OpB;
+    constexpr int STAGE = OpA::BYTES + OpB::BYTES;
+    char* const sb = (char*)smemd;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wmn = WK == 1 ? w : w % (WM * WN), wk = WK == 1 ? 0 : w / (WM * WN);
+    const int wm = wmn / WN, wn = wmn % WN;
+    const int kb = WK == 1 ? 0 : wk * NS * 8;
+
+    d4v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
+
+    const int KT = K / 16;
+    d2v a0[TM], b0[TN], a1[TM], b1[TN];
+    auto frags = [&](const char* stage, int kk, d2v(&a)[TM], d2v(&b)[TN]) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = OpA::frag(stage, wm * TM * 16 + i * 16, kk, lane);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = OpB::frag(stage + OpA::BYTES, wn * TN * 16 + j * 16, kk, lane);
+    };
+    auto mma = [&](const d2v(&a)[TM], const d2v(&b)[TN]) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+    };
+    auto interleave = [&]() {
+#pragma unroll
+        for (int i = 0; i < TM * TN * 2; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load (to LDS)
+        }
+    };
+
+    OpA::issue(A, lda, m0, 0, sb, w, lane);
+    OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    frags(sb, kb, a0, b0);
+    for (int t = 0; t < KT; ++t) {
+        const char* cur = sb + (t & 1) * STAGE;
+        char* nxt = sb + ((t + 1) & 1) * STAGE;
+        const int kn = (t + 1 < KT ? t + 1 : t) * 16;
+        OpA::issue(A, lda, m0, kn, nxt, w, lane);
+        OpB::issue(B, ldb, n0, kn, nxt + OpA::BYTES, w, lane);
+        if constexpr (NS == 2) {
+            frags(cur, kb + 8, a1, b1);
+            mma(a0, b0);
+            interleave();
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
+            __syncthreads();
+            frags(nxt, kb, a0, b0);
+            mma(a1, b1);
+            interleave();
+        } else {
+            mma(a0, b0);
+            interleave();
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            frags(nxt, kb, a1, b1);
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a0[i] = a1[i];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) b0[j] = b1[j];
+        }
+    }
+    if constexpr (WK > 1) __syncthreads();  // the exchange reuses the stage buffers
+    dgemm_finish<WM, WN, TM, TN, WK, false>(acc, C, M, N, ldc, alpha, beta, m0, n0, wm, wn, wmn, wk, lane, smemd);
+}
 
 template <int WM, int WN, int TM, int TN, int BK, int WK, bool TA, bool TB>
 __global__ __launch_bounds__(WM* WN * WK * 64) void dgemm_mfma_kernel(
@@ -123,6 +355,12 @@ __global__ __launch_bounds__(WM* WN * WK * 64) void dgemm_mfma_kernel(
     tile_coords_f64(blockIdx.x, gridDim.x, tiles_m, tiles_n, group_m, tmi, tni);
     const int m0 = tmi * BM, n0 = tni * BN;
     const bool full_mn = vec_ok && (m0 + BM <= M) && (n0 + BN <= N);
+    if constexpr (dgemm_has_dma<WM, WN, TM, TN, BK, WK>()) {
+        if (full_mn && K >= 16 && K % 16 == 0) {
+            dgemm_tile_dma<WM, WN, TM, TN, WK, TA, TB>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, m0, n0, smemd);
+            return;
+        }
+    }
 
     d4v acc[TM][TN];
 #pragma unroll
@@ -178,50 +416,7 @@ __global__ __launch_bounds__(WM* WN * WK * 64) void dgemm_mfma_kernel(
         __syncthreads();
     }
 
-    if constexpr (WK > 1) {
-        // partial sums of group g for owner o != g (the stage buffers are free)
-        auto slot = [&](int g, int o, int i, int j, int r) {
-            const int region = g * (WK - 1) + (o < g ? o : o - 1);
-            return ((((region * WM * WN + wmn) * TM + i) * TN + j) * RS + r % RS) * 64 + lane;
-        };
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (r / RS != wk) smemd[slot(wk, r / RS, i, j, r)] = acc[i][j][r];
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (r / RS == wk) {
-                        double v = wk == 0 ? acc[i][j][r] : smemd[slot(0, wk, i, j, r)];
-#pragma unroll
-                        for (int g = 1; g < WK; ++g) v += g == wk ? acc[i][j][r] : smemd[slot(g, wk, i, j, r)];
-                        acc[i][j][r] = v;
-                    }
-    }
-
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                if (WK > 1 && r / RS != wk) continue;
-                const int row = m0 + wm * TM * 16 + i * 16 + r * 4 + (lane >> 4);
-                const int col = n0 + wn * TN * 16 + j * 16 + (lane & 15);
-                if (row < M && col < N) {
-                    double* p = C + (int64_t)row * ldc + col;
-                    double v = alpha * acc[i][j][r];
-                    if (beta != 0.0) v += beta * *p;
-                    *p = v;
-                }
-            }
+    dgemm_finish<WM, WN, TM, TN, WK, true>(acc, C, M, N, ldc, alpha, beta, m0, n0, wm, wn, wmn, wk, lane, smemd);
 }
 
 // Instantiated configurations: (WM, WN, TM, TN, BK, WK); the WK > 1 entries
@@ -257,7 +452,9 @@ bool mfma_f64_has(const MfmaCfg& c) {
 template <int WM, int WN, int TM, int TN, int BK, int WK>
 static int launch_one64(const GemmArgs& a, hipStream_t st) {
     constexpr int BM = WM * TM * 16, BN = WN * TN * 16;
-    constexpr size_t smem = DgemmShape<WM, WN, TM, TN, BK, WK>::SMEM;
+    constexpr size_t smem_reg = DgemmShape<WM, WN, TM, TN, BK, WK>::SMEM;
+    constexpr size_t smem_dma = dgemm_has_dma<WM, WN, TM, TN, BK, WK>() ? 2 * (size_t)(BM + BN) * 128 : 0;
+    constexpr size_t smem = smem_reg > smem_dma ? smem_reg : smem_dma;
     const int tiles_m = (int)((a.M + BM - 1) / BM), tiles_n = (int)((a.N + BN - 1) / BN);
     const double* A = (const double*)a.A;
     const double* B = (const double*)a.B;

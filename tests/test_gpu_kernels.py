@@ -334,6 +334,38 @@ def test_dgemm_every_cfg(ctx):
             assert cw <= F64_TOL, (cfg, "general", cw)
 
 
+DMA64_CFGS = [c for c in Context.tile_cfgs(_capi.GAT_F64) if c[4] == 16 and c[6] in (1, 2)]
+
+
+@pytest.mark.parametrize("cfg", DMA64_CFGS, ids=str)
+@pytest.mark.parametrize("K", [96, 100])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_dgemm_direct_lds_path(ctx, ta, tb, K, cfg):
+    # FP64 tiles staged 16 deep (no k-split or two groups) stage interior
+    # tiles straight into LDS; edge tiles, K = 100 (not a multiple of 16) and
+    # leading dimensions one element longer (rows not 16-byte aligned) take
+    # the register path.  Against the FP64 oracle, and bit for bit against
+    # the register path on the same data.
+    M, N = 320, 200
+    A = uniform_host(M * K, 37, 0, dtype=np.float64)
+    B = uniform_host(K * N, 37, 1, dtype=np.float64)
+    C0 = uniform_host(M * N, 37, 2, dtype=np.float64)
+    ra, ca = (K, M) if ta else (M, K)
+    rb, cb = (N, K) if tb else (K, N)
+    pad = lambda x, r, c: np.concatenate([x.reshape(r, c), np.zeros((r, 1), np.float64)], axis=1).ravel()
+    dA, dB = ctx.to_device(A), ctx.to_device(B)
+    pA, pB = ctx.to_device(pad(A, ra, ca)), ctx.to_device(pad(B, rb, cb))
+    C64, MAG = ora.gemm64(A, B, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, C0=C0)
+    _say("dgemm direct-LDS", cfg, ta, tb, K)
+    d1, d2 = ctx.to_device(C0), ctx.to_device(C0)
+    ctx.gemm(dA, dB, d1, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, cfg=cfg)
+    ctx.gemm(pA, pB, d2, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, lda=ca + 1, ldb=cb + 1, cfg=cfg)
+    got, ref = d1.download(), d2.download()
+    cw, _, _ = ora.error_metrics(got, C64, MAG)
+    assert cw <= F64_TOL, (cfg, ta, tb, K, cw)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), (cfg, ta, tb, K)
+
+
 def test_sgemm_8192_general_form_sampled(ctx):
     # Config: C = 1.5*A*B - 0.5*C, M=N=K=8192, FP32 row-major, inputs U[-1,1].
     n = 8192
