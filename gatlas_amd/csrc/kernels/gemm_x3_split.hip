@@ -72,38 +72,51 @@ __global__ __launch_bounds__(WM* WN * 64) __attribute__((amdgpu_waves_per_eu(2))
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+    // one 16-deep step's operands: hi and lo planes of the wave's A and B blocks
+    struct Frags {
+        bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+    };
+    Frags f0, f1;
     // 16-deep step s of a stage: hi chunk 2s + lane/32, lo chunk 4 more
-    auto frags = [&](const char* st, int s) {
+    auto frags = [&](const char* st, int s, Frags& f) {
         const int c = 2 * s + (lane >> 5);
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
             const int row = wm * TM * 32 + i * 32 + (lane & 31);
             const u32x4 h = OpA::chunk(st, row, c), l = OpA::chunk(st, row, c + 4);
-            ah[i] = *(const bf16x8*)&h;
-            al[i] = *(const bf16x8*)&l;
+            f.ah[i] = *(const bf16x8*)&h;
+            f.al[i] = *(const bf16x8*)&l;
         }
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
             const int row = wn * TN * 32 + j * 32 + (lane & 31);
             const u32x4 h = OpB::chunk(st + OpA::BYTES, row, c), l = OpB::chunk(st + OpA::BYTES, row, c + 4);
-            bh[j] = *(const bf16x8*)&h;
-            bl[j] = *(const bf16x8*)&l;
+            f.bh[j] = *(const bf16x8*)&h;
+            f.bl[j] = *(const bf16x8*)&l;
         }
     };
-    auto mma3 = [&]() {
+    auto mma3 = [&](const Frags& f) {
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
             }
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int j = 0; j < TN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+    };
+    // one MFMA, then one LDS read and one global load if any remain, ...
+    auto interleave = [&]() {
+#pragma unroll
+        for (int i = 0; i < 3 * TM * TN; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load (to LDS)
+        }
     };
 
     const int KT = kpad / 32;
@@ -111,19 +124,24 @@ __global__ __launch_bounds__(WM* WN * 64) __attribute__((amdgpu_waves_per_eu(2))
     OpB::issue(Bs, ld2, n0, 0, sb + OpA::BYTES, w, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
+    frags(sb, 0, f0);
+    // The fragments of step 1 are read under the MFMAs of step 0; the wait
+    // for the next stage and the barrier come before step 1's MFMAs, which
+    // run under the reads of the next stage's step 0.
     for (int t = 0; t < KT; ++t) {
         const char* cur = sb + (t & 1) * STAGE;
         char* nxt = sb + ((t + 1) & 1) * STAGE;
         const int tn = t + 1 < KT ? t + 1 : t;  // the last iteration fetches its own stage again
         OpA::issue(As, ld2, m0, 64 * tn, nxt, w, lane);
         OpB::issue(Bs, ld2, n0, 64 * tn, nxt + OpA::BYTES, w, lane);
-        __builtin_amdgcn_iglp_opt(0);
-        frags(cur, 0);
-        mma3();
-        frags(cur, 1);
-        mma3();
+        frags(cur, 1, f1);
+        mma3(f0);
+        interleave();
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
         __syncthreads();
+        frags(nxt, 0, f0);
+        mma3(f1);
+        interleave();
     }
 
 #pragma unroll
