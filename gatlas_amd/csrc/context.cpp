@@ -203,11 +203,11 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
         small = MfmaCfg{2, 2, 1, 1, 64, 32};
         tiny = MfmaCfg{1, 1, 2, 2, 64, 32};
     } else {
-        // 256 x 256, 32-deep stages: interior tiles stage straight into LDS
-        // (sgemm_dma_tile.hpp).  NN takes 8 waves of 4 x 2 blocks, the other
-        // transpose pairs 16 waves of 2 x 2 (8192^3 NN 144.3 vs 142.6, TN
-        // 146.9 vs 143.4 TFLOP/s on one box)
-        big = (!a.transA && !a.transB) ? MfmaCfg{2, 4, 4, 2, 32, 32} : MfmaCfg{4, 4, 2, 2, 32, 32};
+        // 256 x 256, 16 waves of 2 x 2 blocks, 32-deep stages: interior
+        // tiles stage straight into LDS (sgemm_dma_tile.hpp).  Interleaved
+        // timing in one process put it 0.6 % above the 8-wave tile at 8192^3
+        // (NN 145.1 vs 144.2, TT 145.7 vs 144.8 TFLOP/s medians).
+        big = MfmaCfg{4, 4, 2, 2, 32, 32};
         mid = MfmaCfg{2, 2, 2, 2, 32, 32};
         small = MfmaCfg{2, 2, 1, 1, 32, 32};
         tiny = MfmaCfg{1, 1, 2, 2, 32, 32};
