@@ -19,7 +19,7 @@
 // Tile shapes: 0 = 128 x 128 (2 x 2 waves of 2 x 2 32x32 blocks),
 //              1 = 64 x 64 (2 x 2 waves of one 32x32 block).
 // BF16 interior tiles of problems with K >= 64 a multiple of 8, and FP32
-// interior tiles of problems with K a multiple of 32, with 16-byte aligned
+// interior tiles of problems with K >= 32 a multiple of 16, with 16-byte aligned
 // rows, stage straight into LDS (bf16_dma_tile.hpp, sgemm_dma_tile.hpp);
 // the others, and every split-BF16x3 tile, stage through registers.  The
 // split-BF16x3 form splits FP32 operands into bf16 hi/lo planes while a
@@ -36,12 +36,13 @@
 
 namespace gat {
 
-// Interior FP32 tiles of problems with K a multiple of 32 and 16-byte
-// aligned rows stage straight into LDS (sgemm_dma_tile.hpp, the same bits).
+// Interior FP32 tiles of problems with K >= 32 a multiple of 16 and
+// 16-byte aligned rows stage straight into LDS (sgemm_dma_tile.hpp, the
+// same bits).
 template <bool TA, bool TB>
 __device__ __forceinline__ void f32_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0, float* smem) {
     const bool vec = (p.flags & GP_VEC) != 0;
-    const bool dma = vec && p.K >= 32 && p.K % 32 == 0;
+    const bool dma = vec && p.K >= 32 && p.K % 16 == 0;
     const float* A = (const float*)p.A;
     const float* B = (const float*)p.B;
     if (shape == 0) {

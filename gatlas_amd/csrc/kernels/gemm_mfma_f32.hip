@@ -2,7 +2,7 @@
 // in XCD-aware supertile order; the tile computation is sgemm_tile
 // (sgemm_tile.hpp), shared with the persistent grouped kernel.  The tiles
 // staged 32 deep (32x32 blocks, no k-split or a two-group one) run their
-// interior tiles (K a multiple of 32, 16-byte aligned rows) on the
+// interior tiles (K >= 32 a multiple of 16, 16-byte aligned rows) on the
 // direct-to-LDS routine (sgemm_dma_tile.hpp), which gives the same bits.
 #include "sgemm_dma_tile.hpp"
 #include "sgemm_tile.hpp"
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(WM* WN * WK * 64) void sgemm_mfma_kernel(
     mfma_tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, group_m, tmi, tni);
     if constexpr (sgemm_has_dma<WM, WN, TM, TN, BK, BLK, WK>()) {
         const int m0 = tmi * S::BM, n0 = tni * S::BN;
-        if (vec_ok && K >= 32 && K % 32 == 0 && m0 + S::BM <= M && n0 + S::BN <= N) {
+        if (vec_ok && K >= 32 && K % 16 == 0 && m0 + S::BM <= M && n0 + S::BN <= N) {
             sgemm_tile_dma<WM, WN, TM, TN, TA, TB, WK>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, m0, n0, smem);
             return;
         }

@@ -54,6 +54,20 @@ struct DmaRowsF32 {
         }
     }
 
+    // The k values [k0, k0 + klen) (klen < 32 a multiple of 4) written into
+    // the last klen k-rows of a stage in `lds`, the first 32 - klen zero,
+    // through registers by all nt work-items: a short first stage.
+    __device__ static __forceinline__ void stage_head(const float* src, int64_t ld, int r0, int k0, int klen,
+                                                      char* lds, int tid, int nt) {
+        for (int q = tid; q < 32 * CPR; q += nt) {
+            const int k = q / CPR, c = q % CPR;
+            const int kr = k - (32 - klen);  // k value held by k-row k, < 0: padding
+            f4v v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (kr >= 0) v = gload<f4v>(src + (int64_t)(k0 + kr) * ld + r0 + 4 * c);
+            *(f4v*)(lds + k * (ROWS * 4) + 16 * (c ^ rsw(k))) = v;
+        }
+    }
+
     // B-side operand of MFMA s of the 8-deep step at kk for row rbase + lane%32:
     // element (k = kk + 4*(lane/32) + s, row)
     __device__ static __forceinline__ void frag(const char* lds, int rbase, int kk, int lane, float (&v)[4]) {
@@ -86,6 +100,19 @@ struct DmaColsF32 {
         }
     }
 
+    // short first stage through registers, k values in the last klen
+    // positions of each row (see DmaRowsF32::stage_head)
+    __device__ static __forceinline__ void stage_head(const float* src, int64_t ld, int r0, int k0, int klen,
+                                                      char* lds, int tid, int nt) {
+        for (int q = tid; q < ROWS * 8; q += nt) {
+            const int row = q >> 3, c = q & 7;
+            const int kr = 4 * c - (32 - klen);  // first k value of chunk c, < 0: padding
+            f4v v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (kr >= 0) v = gload<f4v>(src + (int64_t)(r0 + row) * ld + k0 + kr);
+            *(f4v*)(lds + row * 128 + 16 * (c ^ kswz(row))) = v;
+        }
+    }
+
     // operand of MFMA s of the 8-deep step at kk for row rbase + lane%32:
     // element (row, k = kk + 4*(lane/32) + s)
     __device__ static __forceinline__ void frag(const char* lds, int rbase, int kk, int lane, float (&v)[4]) {
@@ -111,6 +138,10 @@ struct DmaF32Operand<ROWS, true, NW, SW> {
     __device__ static __forceinline__ void frag(const char* lds, int rbase, int kk, int lane, float (&v)[4]) {
         Op::frag(lds, rbase, kk, lane, v);
     }
+    __device__ static __forceinline__ void stage_head(const float* src, int64_t ld, int r0, int k0, int klen,
+                                                      char* lds, int tid, int nt) {
+        Op::stage_head(src, ld, r0, k0, klen, lds, tid, nt);
+    }
 };
 
 template <int ROWS, int NW, int SW>
@@ -124,6 +155,10 @@ struct DmaF32Operand<ROWS, false, NW, SW> {
     __device__ static __forceinline__ void frag(const char* lds, int rbase, int kk, int lane, float (&v)[4]) {
         Op::frag(lds, rbase, kk, lane, v);
     }
+    __device__ static __forceinline__ void stage_head(const float* src, int64_t ld, int r0, int k0, int klen,
+                                                      char* lds, int tid, int nt) {
+        Op::stage_head(src, ld, r0, k0, klen, lds, tid, nt);
+    }
 };
 
 template <int WM, int WN, int TM, int TN, int WK = 1>
@@ -135,9 +170,16 @@ struct SgemmDmaShape {
 };
 
 // Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C for
-// a tile wholly inside M x N, K a positive multiple of 32, 16-byte aligned
+// a tile wholly inside M x N, K >= 32 a multiple of 16, 16-byte aligned
 // operand rows; all WM*WN*WK*64 threads of the group call it; ends after a
-// barrier.  WK > 1 splits each stage's four 8-deep steps between WK groups
+// barrier.  When K is an odd multiple of 16 the first stage holds the 16
+// leading k values in its second half (staged through registers, the first
+// half zero) and every later stage is whole, so stage s covers k
+// [32s - 16, 32s + 16).  The MFMAs pair the same k values as the register
+// path, plus all-zero products ahead of them; with two k-split groups the
+// groups swap stage halves, so each still sums the register path's k values
+// (its groups take [32t, 32t + 16) and [32t + 16, 32t + 32)) in the same
+// order, and the two partial sums commute: the bits agree.  WK > 1 splits each stage's four 8-deep steps between WK groups
 // of WM x WN waves as the register-staged k-split tile does (group g takes
 // steps g*4/WK ..), with the same partial-sum exchange, so the bits agree.
 template <int WM, int WN, int TM, int TN, bool TA, bool TB, int WK = 1>
@@ -160,7 +202,9 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wmn = WK == 1 ? w : w % (WM * WN), wk = WK == 1 ? 0 : w / (WM * WN);
     const int wm = wmn / WN, wn = wmn % WN;
-    const int kb = WK == 1 ? 0 : wk * NS * 8;  // this k-group's run of steps in a stage
+    const int kh = K % 32;  // k values of a short first stage (0: none)
+    // this k-group's run of steps in a stage (halves swapped with a short first stage)
+    const int kb = WK == 1 ? 0 : (kh ? (WK - 1 - wk) : wk) * NS * 8;
 
     f16v acc[TM][TN];
 #pragma unroll
@@ -170,7 +214,8 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    const int KT = K / 32;
+    const int KT = K / 32 + (kh != 0);
+    auto kbase = [&](int st) { return st == 0 ? 0 : st * 32 - (kh ? 32 - kh : 0); };  // first k of stage st
     float a0[TM][4], b0[TN][4], a1[TM][4], b1[TN][4];
     auto frags = [&](const char* stage, int kk, float(&a)[TM][4], float(&b)[TN][4]) {
 #pragma unroll
@@ -199,8 +244,13 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
         }
     };
 
-    OpA::issue(A, lda, m0, 0, sb, w, lane);
-    OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
+    if (kh) {
+        OpA::stage_head(A, lda, m0, 0, kh, sb, threadIdx.x, NW * 64);
+        OpB::stage_head(B, ldb, n0, 0, kh, sb + OpA::BYTES, threadIdx.x, NW * 64);
+    } else {
+        OpA::issue(A, lda, m0, 0, sb, w, lane);
+        OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
+    }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     frags(sb, kb, a0, b0);
@@ -214,7 +264,7 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
     for (int t = 0; t < KT; ++t) {
         const char* cur = sb + (t & 1) * STAGE;
         char* nxt = sb + ((t + 1) & 1) * STAGE;
-        const int kn = (t + 1 < KT ? t + 1 : t) * 32;
+        const int kn = kbase(t + 1 < KT ? t + 1 : t);  // K >= 32: the last stage is whole
         OpA::issue(A, lda, m0, kn, nxt, w, lane);
         OpB::issue(B, ldb, n0, kn, nxt + OpA::BYTES, w, lane);
         if constexpr (NS == 4) {

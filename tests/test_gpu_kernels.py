@@ -248,15 +248,16 @@ DMA_CFGS = [c for c in Context.tile_cfgs(_capi.GAT_F32) if c[4] == 32 and c[5] =
 
 
 @pytest.mark.parametrize("cfg", DMA_CFGS, ids=str)
-@pytest.mark.parametrize("K", [96, 100])
+@pytest.mark.parametrize("K", [96, 112, 100])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_sgemm_direct_lds_path(ctx, ta, tb, K, cfg):
     # Tiles staged 32 deep (no k-split, or two k-split groups) stage interior
     # tiles straight from HBM into LDS (k-contiguous operands XOR-swizzled,
     # row-contiguous ones as [k][row] with a half-bank shift every four
     # k-rows) and edge tiles through registers.  M, N give whole and partial
-    # 256-wide tiles; K = 100 is not a multiple of 32, so every tile takes the
-    # register path.  Checked against the FP64 oracle, and bit for bit
+    # 256-wide tiles; K = 112 adds a short first stage of 16 k values staged
+    # through registers; K = 100 is not a multiple of 16, so every tile takes
+    # the register path.  Checked against the FP64 oracle, and bit for bit
     # against the register path on the same data (leading dimensions one
     # element longer, not 16-byte multiples, force that path on every tile).
     M, N = 512, 384
