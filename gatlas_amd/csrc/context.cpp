@@ -182,10 +182,16 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
     };
     auto fills = [&](const MfmaCfg& c) { return tiles(c) >= kFillTiles; };
     if (dtype == DT_F64) {
-        // 64 x 64 tiles of 16x16 blocks, always k-split: four wave groups
-        // (32-deep stages) below two work-groups per CU, else two (16-deep).
-        // Two groups beat the unsplit tile at every size of the sweep
-        // (1024^3 .. 8192^3: 42 vs 33, 56 vs 48, 60 vs 54, 61 vs 58 TFLOP/s).
+        // Eight waves of 2 x 2 16x16 blocks on a 64 x 128 tile (128 x 64
+        // when A is transposed: its k-rows are then M-contiguous) wherever
+        // that gives two work-groups per CU.  Interleaved timing in one
+        // process, general form, against the two-group 64 x 64 tile:
+        // 8192^3 NN 70.1 vs 62.6, TT 67.4 vs 63.6; 4096^3 NN 68.7 vs 61.8;
+        // 2048^3 NN 65.3 vs 57.7 TFLOP/s.  Smaller problems keep 64 x 64
+        // k-split tiles: four wave groups (32-deep stages) below two
+        // work-groups per CU, else two (16-deep).
+        const MfmaCfg wide = a.transA ? MfmaCfg{4, 2, 2, 2, 16, 16} : MfmaCfg{2, 4, 2, 2, 16, 16};
+        if (tiles(wide) >= 2 * kFillTiles) return wide;
         const MfmaCfg two{2, 2, 2, 2, 16, 16, 2};
         return tiles(two) < 2 * kFillTiles ? MfmaCfg{2, 2, 2, 2, 32, 16, 4} : two;
     }

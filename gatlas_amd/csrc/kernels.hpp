@@ -36,7 +36,7 @@ struct GemmArgs {
 // the macro tile is (waves_m*tm*blk) x (waves_n*tn*blk) and the LDS K depth bk.
 struct MfmaCfg {
     int waves_m, waves_n, tm, tn, bk, blk;
-    int wk = 1;  // wave groups sharing each staged K tile (FP32 tiles only)
+    int wk = 1;  // wave groups sharing each staged K tile (FP32 and FP64 tiles only)
 };
 
 // Reference-geometry ("compat") GEMM: work-group gs x gs work-items, each

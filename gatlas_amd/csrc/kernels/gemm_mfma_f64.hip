@@ -430,7 +430,10 @@ __global__ __launch_bounds__(WM* WN * WK * 64) void dgemm_mfma_kernel(
     X(1, 1, 2, 2, 32, 1)     \
     X(2, 2, 2, 2, 16, 2)     \
     X(2, 2, 2, 2, 32, 2)     \
-    X(2, 2, 2, 2, 32, 4)
+    X(2, 2, 2, 2, 32, 4)     \
+    X(4, 4, 2, 2, 16, 1)     \
+    X(4, 2, 2, 2, 16, 1)     \
+    X(2, 4, 2, 2, 16, 1)
 
 static const MfmaCfg kF64Cfgs[] = {
 #define X(wm, wn, tm, tn, bk, wk) {wm, wn, tm, tn, bk, 16, wk},
