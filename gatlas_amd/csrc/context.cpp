@@ -171,8 +171,11 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
 // of four waves (one wave when a side is below 256).  FP32 64 x 64 tiles
 // that give fewer than two work-groups per CU take their k-split form (two
 // wave groups share each K stage), which keeps more than one wave per SIMD.
-// At 1024^3, 2048^3 and 4096^3 FP32 this picks the same tiles as an
-// exhaustive timing of the table.
+// At 1024^3 FP32 this is the tile an exhaustive timing of the table picks;
+// at 2048^3 and 4096^3 the table timing picks eight-wave tiles of the same
+// size, within the spread of repeated timings (2048^3: 132 vs 125, 4096^3:
+// 147.0 vs 146.6 TFLOP/s in one run; interleaved runs rank 2048^3 the
+// other way round, 127.4 vs 126.1).
 MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
     constexpr int64_t kFillTiles = 256;
     const int64_t batch = a.batch > 1 ? a.batch : 1;
@@ -214,14 +217,17 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
         // timing in one process put it 0.6 % above the 8-wave tile at 8192^3
         // (NN 145.1 vs 144.2, TT 145.7 vs 144.8 TFLOP/s medians).
         big = MfmaCfg{4, 4, 2, 2, 32, 32};
-        mid = MfmaCfg{2, 2, 2, 2, 32, 32};
+        // 128 x 128, 16 waves of one block each: at 2048^3 one tile per CU,
+        // and 16 waves hide the stage loads that 4 waves leave exposed
+        // (interleaved timing, general form: 127.0-128.7 TFLOP/s over the
+        // four transpose pairs against 119.0-121.0 for four waves of 2 x 2
+        // blocks; 3072^3 102 vs 99)
+        mid = MfmaCfg{4, 4, 1, 1, 32, 32};
         small = MfmaCfg{2, 2, 1, 1, 32, 32};
         tiny = MfmaCfg{1, 1, 2, 2, 32, 32};
     }
     const bool split = dtype == DT_F32;
     if (fills(big)) return big;
-    // the 128 x 128 tile stages interior tiles straight into LDS in its
-    // unsplit form, which the tuner prefers at 2048^3 (124 vs 116 TFLOP/s)
     if (fills(mid)) return mid;
     if (fills(small) || mn >= 256)
         return split && tiles(small) < 2 * kFillTiles ? MfmaCfg{2, 2, 1, 1, 32, 32, 2} : small;

@@ -70,7 +70,10 @@ __global__ __launch_bounds__(WM* WN * WK * 64) void sgemm_mfma_kernel(
     X(2, 2, 1, 1, 32, 32, 2)  \
     X(2, 2, 1, 1, 64, 32, 4)  \
     X(2, 2, 2, 2, 32, 32, 2)  \
-    X(2, 2, 2, 2, 64, 32, 2)
+    X(2, 2, 2, 2, 64, 32, 2)  \
+    X(4, 4, 1, 1, 32, 32, 1)  \
+    X(2, 4, 2, 1, 32, 32, 1)  \
+    X(4, 2, 1, 2, 32, 32, 1)
 
 static const MfmaCfg kF32Cfgs[] = {
 #define X(wm, wn, tm, tn, bk, blk, wk) {wm, wn, tm, tn, bk, blk, wk},
