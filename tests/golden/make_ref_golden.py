@@ -3,7 +3,7 @@
 helpers by running the reference's own code (oracle/_ref/ref_driver, built
 from the reference sources in place by oracle/Makefile.ref).
 
-Run in the build container (the reference checkout is not on the GPU box):
+Run where the reference checkout is present (the tests read only the fixtures):
     python tests/golden/make_ref_golden.py
 Outputs (committed): ref_params.jsonl, ref_journal.json, ref_rank.jsonl.
 """
