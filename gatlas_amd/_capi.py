@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libgatlas_hip.so")
 GAT_F32, GAT_F64, GAT_BF16, GAT_BF16X3, GAT_FP8 = 0, 1, 2, 3, 4
 GAT_MATMUL, GAT_MATVEC, GAT_SAXPY = 0, 1, 2
 GAT_BUFFER, GAT_IMAGE, GAT_MFMA, GAT_NATIVE = 0, 1, 2, 3
+GAT_OK, GAT_ERR_INVALID, GAT_ERR_HIP, GAT_ERR_NOMEM, GAT_ERR_UNSUPPORTED = 0, -1, -2, -3, -4
 
 _c_sz = ctypes.c_size_t
 _c_vp = ctypes.c_void_p

@@ -210,6 +210,10 @@ int gat_gemm_strided(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, 
                      const void* A, size_t lda, size_t strideA, const void* B, size_t ldb, size_t strideB,
                      double beta, void* C, size_t ldc, size_t strideC, size_t batch) {
     CTX_CHECK(ctx);
+    if (batch > (size_t)INT32_MAX) {
+        set_error("gemm: batch above 2^31 - 1");
+        return GAT_ERR_INVALID;
+    }
     GemmArgs a = make_args(transA, transB, M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
                            strideC, batch);
     int rc = check_gemm_shape(a);
@@ -341,6 +345,12 @@ int gat_quantize_fp8(gat_ctx* ctx, const float* src, size_t rows, size_t cols, s
 int gat_gemm_fp8(gat_ctx* ctx, size_t M, size_t N, size_t K, double alpha, const void* A8, const float* scaleA,
                  const void* B8, const float* scaleB, double beta, float* C, size_t ldc) {
     CTX_CHECK(ctx);
+    // resident e4m3 operands always come with their block scales (without
+    // them the FP32-operand path would read the byte arrays as floats)
+    if (M && N && K && (!A8 || !B8 || !scaleA || !scaleB)) {
+        set_error("gemm_fp8: null operand or block scales");
+        return GAT_ERR_INVALID;
+    }
     GemmArgs a = make_args(0, 1, M, N, K, alpha, A8, K, 0, B8, K, 0, beta, C, ldc, 0, 1);
     a.scaleA = scaleA;
     a.scaleB = scaleB;

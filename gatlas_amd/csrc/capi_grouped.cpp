@@ -134,7 +134,7 @@ int gat_grouped_tune(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t
                 best_shape = s;
             }
         }
-        c.set_tuned(kv.first, GroupedPlan::shape_cfg(dtype, best_shape));
+        c.set_grouped_shape(kv.first, best_shape);
         ++tuned;
     }
     return tuned;

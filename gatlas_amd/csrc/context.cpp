@@ -14,7 +14,7 @@ static int ilog2_clamp(int64_t v, int lo, int hi) {
 }
 
 BucketKey bucket_of(int dtype, const GemmArgs& a) {
-    return BucketKey{dtype, a.transA, a.transB, ilog2_clamp(a.M, 4, 15), ilog2_clamp(a.N, 4, 15),
+    return BucketKey{dtype, a.transA ? 1 : 0, a.transB ? 1 : 0, ilog2_clamp(a.M, 4, 15), ilog2_clamp(a.N, 4, 15),
                      ilog2_clamp(a.K, 4, 15)};
 }
 
@@ -126,9 +126,23 @@ bool Context::get_tuned(const BucketKey& k, MfmaCfg* c) const {
     return true;
 }
 
+void Context::set_grouped_shape(const BucketKey& k, int shape) {
+    std::lock_guard<std::mutex> g(mu_);
+    grouped_shape_[k] = shape;
+}
+
+bool Context::get_grouped_shape(const BucketKey& k, int* shape) const {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = grouped_shape_.find(k);
+    if (it == grouped_shape_.end()) return false;
+    *shape = it->second;
+    return true;
+}
+
 void Context::clear_tuned() {
     std::lock_guard<std::mutex> g(mu_);
     tuned_.clear();
+    grouped_shape_.clear();
 }
 
 int Context::num_cfgs(int dtype) {
@@ -171,6 +185,9 @@ bool Context::has_cfg(int dtype, const MfmaCfg& c) {
 // of four waves (one wave when a side is below 256).  FP32 64 x 64 tiles
 // that give fewer than two work-groups per CU take their k-split form (two
 // wave groups share each K stage), which keeps more than one wave per SIMD.
+// BF16 departs from the largest-tile rule below the 256 x 256 tile: it takes
+// 64 x 64 tiles wherever 128 x 128 ones give fewer than two work-groups per
+// CU.  FP64 and FP8 have rules of their own (in the function body).
 // At 1024^3 FP32 this is the tile an exhaustive timing of the table picks;
 // at 2048^3 and 4096^3 the table timing picks eight-wave tiles of the same
 // size, within the spread of repeated timings (2048^3: 132 vs 125, 4096^3:

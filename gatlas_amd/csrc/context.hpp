@@ -13,7 +13,8 @@
 namespace gat {
 
 // Size bucket of a GEMM problem for tile selection: log2-rounded M, N, K
-// (clamped), plus the transpose pair and the precision form.
+// (clamped), plus the transpose pair (any non-zero flag counts as 1) and the
+// precision form.
 struct BucketKey {
     int dtype, ta, tb, lm, ln, lk;
     bool operator<(const BucketKey& o) const {
@@ -57,6 +58,11 @@ public:
     // Tuned tile choice per size bucket (filled by the autotuner).
     void set_tuned(const BucketKey& k, const MfmaCfg& c);
     bool get_tuned(const BucketKey& k, MfmaCfg* c) const;
+    // Grouped-kernel tile shape per size bucket (filled by the grouped
+    // tuner; a table of its own, so single-problem GEMMs keep theirs).
+    void set_grouped_shape(const BucketKey& k, int shape);
+    bool get_grouped_shape(const BucketKey& k, int* shape) const;
+    // Forgets both tables.
     void clear_tuned();
     static MfmaCfg default_cfg(int dtype, const GemmArgs& a);
     static int num_cfgs(int dtype);
@@ -70,6 +76,7 @@ private:
     void* work_ = nullptr;
     size_t work_bytes_ = 0;
     std::map<BucketKey, MfmaCfg> tuned_;
+    std::map<BucketKey, int> grouped_shape_;
     mutable std::mutex mu_;
 };
 

@@ -38,8 +38,8 @@ int GroupedPlan::choose_shape(const Context& ctx, int dtype, const GroupedDesc& 
     a.M = d.M;
     a.N = d.N;
     a.K = d.K;
-    MfmaCfg c;
-    if (ctx.get_tuned(bucket_of(dtype, a), &c)) return (c.waves_m * c.tm * c.blk >= 128) ? 0 : 1;
+    int s = 0;
+    if (ctx.get_grouped_shape(bucket_of(dtype, a), &s)) return s;
     // untuned: 128 x 128 tiles once both sides fill them, else 64 x 64
     return (d.M >= 256 && d.N >= 256) ? 0 : 1;
 }

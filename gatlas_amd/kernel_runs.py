@@ -104,86 +104,14 @@ def run(ctx, name, reps):
     print("[kernel_runs] %s x%d done" % (name, reps), flush=True)
 
 
-def sweep_tiles(ctx, n, forms, reps):
-    """Every instantiated tile configuration of each form at n^3 (C = A*B, no
-    transpose): one warm-up, then `reps` launches timed with HIP events."""
-    codes = {"f32": _capi.GAT_F32, "f64": _capi.GAT_F64, "bf16": _capi.GAT_BF16, "bf16x3": _capi.GAT_BF16X3,
-             "fp8": _capi.GAT_FP8}
-    print("| form | tile | ms | TFLOP/s |\n|---|---|---|---|", flush=True)
-    for form in forms:
-        dt = codes[form]
-        f = ctx.empty(n * n, np.float64 if dt == _capi.GAT_F64 else np.float32)
-        g = ctx.empty(n * n, f.dtype)
-        C = ctx.empty(n * n, f.dtype)
-        ctx.fill_uniform(f, 1, 0)
-        ctx.fill_uniform(g, 1, 1)
-        A, B = f, g
-        if dt == _capi.GAT_BF16:
-            A, B = ctx.empty(n * n, np.uint16), ctx.empty(n * n, np.uint16)
-            ctx.convert_bf16(f, A)
-            ctx.convert_bf16(g, B)
-        for cfg in Context.tile_cfgs(dt):
-            ctx.gemm(A, B, C, n, n, n, dtype=dt, cfg=cfg)
-            ctx.sync()
-            ctx.timer_start()
-            for _ in range(reps):
-                ctx.gemm(A, B, C, n, n, n, dtype=dt, cfg=cfg)
-            ms = ctx.timer_stop() / reps
-            print("| %s | %s | %.3f | %.1f |" % (form, cfg, ms, 2.0 * n ** 3 / ms / 1e9), flush=True)
-
-
-def sweep_bandwidth(ctx, reps=20):
-    """Launch shapes of the bandwidth kernels through the parameter interface
-    (Bench::run, `reps` chained launches timed on the host): GEMV y = A*x,
-    16384^2 (work-group, rows, unroll digit; rows = rows per wave, or rows
-    of A in flight / 2 in the transposed form, whose unroll digit sets the
-    work-group count) and SAXPY z = 1.5x + y, n = 2^28 (work-group, vectors
-    per work-item, non-temporal digit)."""
-    from .interface import MATVEC, NATIVE, SAXPY, Kernel
-    m, n = 16384, 1 << 28
-    k = Kernel(MATVEC, _capi.GAT_F32, 4, NATIVE, ctx=ctx)
-    print("| kernel | params | us | GB/s |\n|---|---|---|---|", flush=True)
-    for tr in (0, 1):
-        for wg in (64, 128, 256, 512):
-            for rows in (1, 2, 4, 8):
-                for xp in (0, 1, 2):
-                    p = [1, 0, m, m, tr, wg, rows, 4, xp]
-                    k.setParams(p)
-                    if not k.validParams():
-                        continue
-                    k.setArgs(True)
-                    us = k.run(reps) / reps
-                    print("| gemv%s | %s | %.1f | %.0f |" % ("T" if tr else "", p, us,
-                                                           4.0 * (m * m + 2 * m) / max(us, 1e-3) / 1e3), flush=True)
-    s = Kernel(SAXPY, _capi.GAT_F32, 4, NATIVE, ctx=ctx)
-    for wg in (64, 128, 256, 512, 1024):
-        for un in (1, 2, 4, 8):
-            for nt in (0, 1):
-                p = [1, n // 4, 4, wg, 0, un, 0, 4, nt]
-                s.setParams(p)
-                if not s.validParams():
-                    continue
-                s.setArgs(True)
-                us = s.run(reps) / reps
-                print("| saxpy | %s | %.1f | %.0f |" % (p, us, 12.0 * n / max(us, 1e-3) / 1e3), flush=True)
-
-
 def main(argv=None):
     p = argparse.ArgumentParser()
     p.add_argument("--reps", type=int, default=3)
-    p.add_argument("--sweep-tiles", type=int, default=0, metavar="N",
-                   help="time every tile configuration of the given forms at N^3")
-    p.add_argument("--sweep-bandwidth", action="store_true", help="launch-shape sweep of GEMV and SAXPY")
-    p.add_argument("workloads", nargs="*", help="workload names, or forms with --sweep-tiles")
+    p.add_argument("workloads", nargs="*", help="workload names")
     a = p.parse_args(argv)
     ctx = Context(0)
-    if a.sweep_bandwidth:
-        sweep_bandwidth(ctx)
-    elif a.sweep_tiles:
-        sweep_tiles(ctx, a.sweep_tiles, a.workloads, a.reps)
-    else:
-        for w in a.workloads:
-            run(ctx, w, a.reps)
+    for w in a.workloads:
+        run(ctx, w, a.reps)
     ctx.close()
     return 0
 

@@ -213,7 +213,8 @@ int gat_gemm_grouped(gat_ctx *ctx, int dtype, const gat_gemm_desc *descs, size_t
 /* Times both tile shapes for one problem per size bucket, inside a full
  * persistent launch of replicas of that problem (reading its A and B, writing
  * library scratch: the caller's C is not touched), and installs the faster in
- * the context's tuned table; returns the number of buckets tuned. */
+ * the context's grouped-shape table (single-problem GEMMs keep their own
+ * tuned tiles); returns the number of buckets tuned. */
 int gat_grouped_tune(gat_ctx *ctx, int dtype, const gat_gemm_desc *descs, size_t count, int reps);
 
 /* Grouped GEMV (FP32, A row-major M x N): y = alpha*A*x + beta*y for many
@@ -409,7 +410,8 @@ int gat_tuned_set(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, siz
                   const gat_tile_cfg *cfg);
 int gat_tuned_get(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, size_t N, size_t K,
                   gat_tile_cfg *out);
-/* Forget every tuned choice: all shapes use the default tile rule again. */
+/* Forget every tuned choice (single-problem tiles and grouped shapes): all
+ * shapes use the default tile rules again. */
 int gat_tuned_clear(gat_ctx *ctx);
 
 #ifdef __cplusplus

@@ -12,6 +12,8 @@
     multiples of 16, random transposes): bit-identical to the per-problem
     launch loop, a row sample of every problem against the FP64 oracle.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -182,11 +184,21 @@ def test_grouped_512_workload(ctx, dtype):
         dC = ctx.empty(M * N)
         probs.append(dict(transA=ta, transB=tb, M=M, N=N, K=K, alpha=1.0, beta=0.0, A=dA, B=dB, C=dC))
     assert esz in (2, 4)
-    ctx.grouped_tune(probs, dtype)
-    g = ctx.grouped(probs, dtype, use_graph=True)
-    g.run()
-    ctx.sync()
-    shapes = g.shapes()
+    single_before = {d: ctx.tile_for(d[0], d[1], d[2], bool(d[3]), bool(d[4]), dtype) for d in dims[:32]}
+    try:
+        ctx.grouped_tune(probs, dtype)
+        # the grouped tuner keeps a table of its own: single-problem GEMMs in
+        # the same size buckets keep their (default) tiles
+        for d, cfg in single_before.items():
+            c = _capi.TileCfg()
+            assert ctx.lib.gat_tuned_get(ctx.h, dtype, d[3], d[4], d[0], d[1], d[2], ctypes.byref(c)) == 0, d
+            assert c.as_tuple() == cfg, d
+        g = ctx.grouped(probs, dtype, use_graph=True)
+        g.run()
+        ctx.sync()
+        shapes = g.shapes()
+    finally:
+        ctx.tuned_clear()
     got = [p["C"].download() for p in probs]
     # per-problem launch loop with the same tile choice: bit-identical
     ref = ctx.empty(max(m * n for m, n, _, _, _ in dims))
