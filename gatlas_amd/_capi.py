@@ -41,6 +41,7 @@ class GemvDesc(ctypes.Structure):
                 ("beta", _c_dbl), ("y", _c_vp)]
 
 
+GAT_GEMV_NOALLOC_SCRATCH = 16 << 20   # bytes of static scratch per GPU (gatlas.h)
 GAT_SERVE_BATCHED = 0
 GAT_SERVE_LOOP = 1
 
@@ -122,6 +123,8 @@ SIGNATURES = {
     "gat_gemm_compat": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_sz, _c_sz, _c_sz, _c_int, _c_dbl, _c_vp,
                                  _c_vp, _c_dbl, _c_vp, _c_sz, _c_sz, _c_sz, _c_sz]),
     "gat_gemv": (_c_int, [_c_vp, _c_int, _c_int, _c_sz, _c_sz, _c_dbl, _c_vp, _c_sz, _c_vp, _c_dbl, _c_vp]),
+    "gat_gemv_noalloc": (_c_int, [_c_vp, _c_int, _c_int, _c_sz, _c_sz, _c_dbl, _c_vp, _c_sz, _c_vp, _c_dbl,
+                                  _c_vp]),
     "gat_gemv_compat": (_c_int, [_c_vp, _c_int, _c_int, _c_sz, _c_sz, _c_int, _c_dbl, _c_vp, _c_vp, _c_dbl,
                                  _c_vp, _c_sz, _c_sz, _c_sz, _c_sz]),
     "gat_axpy": (_c_int, [_c_vp, _c_int, _c_sz, _c_dbl, _c_vp, _c_vp, _c_vp]),

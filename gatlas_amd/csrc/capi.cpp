@@ -292,6 +292,22 @@ int gat_gemv(gat_ctx* ctx, int dtype, int transA, size_t M, size_t N, double alp
     return launch_gemv_native(dtype, transA, M, N, alpha, A, lda, x, beta, y, work, ctx->impl.stream);
 }
 
+int gat_gemv_noalloc(gat_ctx* ctx, int dtype, int transA, size_t M, size_t N, double alpha, const void* A,
+                     size_t lda, const void* x, double beta, void* y) {
+    CTX_CHECK(ctx);
+    if ((M && !y) || (M && N && (!A || !x))) {
+        set_error("gemv_noalloc: null operand");
+        return GAT_ERR_INVALID;
+    }
+    if (lda < (transA ? M : N)) {
+        set_error("gemv_noalloc: leading dimension smaller than the row length");
+        return GAT_ERR_INVALID;
+    }
+    ctx->impl.make_current();
+    return launch_gemv_noalloc(ctx->impl.device, dtype, transA ? 1 : 0, M, N, alpha, A, lda, x, beta, y,
+                               ctx->impl.stream);
+}
+
 int gat_gemv_compat(gat_ctx* ctx, int dtype, int transA, size_t M, size_t N, int gemv, double alpha,
                     const void* A, const void* x, double beta, void* y, size_t gs, size_t bh, size_t vl,
                     size_t packed) {

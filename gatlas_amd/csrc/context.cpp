@@ -49,7 +49,8 @@ int Context::init(int dev) {
     GAT_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     GAT_HIP_TRY(hipEventCreate(&ev_start));
     GAT_HIP_TRY(hipEventCreate(&ev_stop));
-    return ST_OK;
+    // device address of the allocation-free GEMV's static scratch (once per device)
+    return gemv_scratch_init(dev);
 }
 
 void Context::make_current() const { hipSetDevice(device); }

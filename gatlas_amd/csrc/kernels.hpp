@@ -82,6 +82,15 @@ size_t gemv_workspace_bytes(int dtype, int transA, size_t M, size_t N);
 int launch_gemv_native(int dtype, int transA, size_t M, size_t N, double alpha, const void* A,
                        size_t lda, const void* x, double beta, void* y, void* work, hipStream_t st,
                        const GemvCfg* cfg = nullptr);
+// Allocation-free form: the transposed partials go to a static per-device
+// array of GEMV_SCRATCH_BYTES (its address taken by gemv_scratch_init, once
+// per device, with that device current); shapes whose partials
+// (gemv_partial_bytes) exceed it are rejected.  Same kernels, same results.
+constexpr size_t GEMV_SCRATCH_BYTES = (size_t)16 << 20;
+int gemv_scratch_init(int device);
+size_t gemv_partial_bytes(int dtype, int transA, size_t M, size_t N);
+int launch_gemv_noalloc(int device, int dtype, int transA, size_t M, size_t N, double alpha, const void* A,
+                        size_t lda, const void* x, double beta, void* y, hipStream_t st);
 // Reference geometry (KernelMatvecBuffer.hpp:180-325): each work-item owns bh
 // rows, j-ascending unfused chain per row.
 int launch_gemv_compat(int dtype, int vl, int transA, size_t M, size_t N, size_t gs, size_t bh,

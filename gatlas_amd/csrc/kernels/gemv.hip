@@ -11,6 +11,8 @@
 // Compat (reference geometry, KernelMatvecBuffer.hpp:180-325): work-item g
 //   owns rows g*bh .. g*bh+bh-1, each a j-ascending chain of separately
 //   rounded multiply and add -- the arithmetic of the reference's host check.
+#include <mutex>
+
 #include "gemv_grouped.hpp"
 #include "gemv_rows.hpp"
 
@@ -162,17 +164,18 @@ static int launch_native_t(int transA, const GemvCfg& cfg, size_t M, size_t N, T
     return ST_OK;
 }
 
+// Default launch shapes: the fastest of the 16384^2 sweep.  Rows: 128
+// work-items, 2 rows per wave, one load per row in flight (the grouped
+// kernel uses the same rows/unroll so results match bit for bit).
+// Transposed: 64 work-items (256 outputs) per group, 8 rows of A in flight
+// per work-item, 1024 work-groups.
+static constexpr GemvCfg kColsDefault{64, 4, 1};
+static constexpr GemvCfg kRowsDefault{128, GEMV_GROUPED_ROWS, GEMV_GROUPED_UNROLL};
+
 int launch_gemv_native(int dtype, int transA, size_t M, size_t N, double alpha, const void* A, size_t lda,
                        const void* x, double beta, void* y, void* work, hipStream_t st, const GemvCfg* cfg) {
     if (M == 0) return ST_OK;
-    // default launch shapes: the fastest of the 16384^2 sweep.  Rows: 128
-    // work-items, 2 rows per wave, one load per row in flight (the grouped
-    // kernel uses the same rows/unroll so results match bit for bit).
-    // Transposed: 64 work-items (256 outputs) per group, 8 rows of A in
-    // flight per work-item, 1024 work-groups.
-    const GemvCfg c = cfg ? *cfg
-                          : transA ? GemvCfg{64, 4, 1}
-                                   : GemvCfg{128, GEMV_GROUPED_ROWS, GEMV_GROUPED_UNROLL};
+    const GemvCfg c = cfg ? *cfg : transA ? kColsDefault : kRowsDefault;
     if (dtype == DT_F32)
         return launch_native_t<float, f4v>(transA, c, M, N, (float)alpha, (const float*)A, lda, (const float*)x,
                                            (float)beta, (float*)y, work, st);
@@ -181,6 +184,70 @@ int launch_gemv_native(int dtype, int transA, size_t M, size_t N, double alpha, 
                                             (double*)y, work, st);
     set_error("gemv: dtype must be f32 or f64");
     return ST_INVALID;
+}
+
+// ---------------- allocation-free entry (static scratch) ----------------
+// The transposed form's slice partials, for callers that cannot allocate:
+// one file-scope array per GPU, held until the process ends.  Host code
+// never passes `scratch` itself (its host-side name is a stand-in, not a
+// device address); each device's address comes from hipGetSymbolAddress,
+// taken once per device with that device current, and only that address
+// reaches a kernel.  Every partial the second pass reads is written by the
+// first pass of the same call, so nothing needs clearing between calls;
+// calls that overlap on one GPU (two streams) share the array and race.
+static __device__ __align__(256) unsigned char scratch[GEMV_SCRATCH_BYTES];
+
+static constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxScratchDevices];
+
+int gemv_scratch_init(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) {
+        set_error("gemv scratch: device index out of range");
+        return ST_INVALID;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return ST_OK;
+    void* p = nullptr;
+    GAT_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+        set_error("gemv scratch: hipGetSymbolAddress returned a null address");
+        return ST_HIP;
+    }
+    g_scratch_ptr[device] = p;
+    return ST_OK;
+}
+
+size_t gemv_partial_bytes(int dtype, int transA, size_t M, size_t N) {
+    if (!transA || M == 0) return 0;
+    const int W = dtype == DT_F64 ? 2 : 4;
+    const int S = gemv_cols_slices(M, N, W, kColsDefault.wg, (size_t)1024 * kColsDefault.unroll);
+    return (size_t)S * M * (dtype == DT_F64 ? 8 : 4);
+}
+
+int launch_gemv_noalloc(int device, int dtype, int transA, size_t M, size_t N, double alpha, const void* A,
+                        size_t lda, const void* x, double beta, void* y, hipStream_t st) {
+    if (dtype != DT_F32 && dtype != DT_F64) {
+        set_error("gemv: dtype must be f32 or f64");
+        return ST_INVALID;
+    }
+    void* work = nullptr;
+    const size_t need = gemv_partial_bytes(dtype, transA, M, N);
+    if (need) {
+        if (need > GEMV_SCRATCH_BYTES) {
+            set_error("gemv_noalloc: the transposed form needs " + std::to_string(need) +
+                      " bytes of partial sums, above the static scratch of " +
+                      std::to_string((size_t)GEMV_SCRATCH_BYTES) + " bytes");
+            return ST_INVALID;
+        }
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        work = (device >= 0 && device < kMaxScratchDevices) ? g_scratch_ptr[device] : nullptr;
+        if (!work) {
+            set_error("gemv_noalloc: static scratch not set up on this device");
+            return ST_INVALID;
+        }
+    }
+    return launch_gemv_native(dtype, transA, M, N, alpha, A, lda, x, beta, y, work, st);
 }
 
 // ---------------- reference geometry ----------------

@@ -165,6 +165,20 @@ int gat_gemm_fp8(gat_ctx *ctx, size_t M, size_t N, size_t K, double alpha, const
 /* y = alpha*op(A)*x + beta*y; op(A) is M x N (A stored N x M when transA). */
 int gat_gemv(gat_ctx *ctx, int dtype, int transA, size_t M, size_t N, double alpha, const void *A, size_t lda,
              const void *x, double beta, void *y);
+/* Allocation-free gat_gemv.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * the same bits as gat_gemv; it never allocates, grows or frees memory.  The
+ * transposed form keeps its slice partial sums in a static device array of
+ * GAT_GEMV_NOALLOC_SCRATCH bytes per GPU (held until the process ends)
+ * instead of the context workspace, and rejects with GAT_ERR_INVALID a shape
+ * whose partials exceed it (about M <= 4M rows for FP32, 2M for FP64, fewer
+ * for short-and-wide shapes); the row-major form needs no scratch.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (several contexts or streams at once) race. */
+#define GAT_GEMV_NOALLOC_SCRATCH ((size_t)16 << 20)
+int gat_gemv_noalloc(gat_ctx *ctx, int dtype, int transA, size_t M, size_t N, double alpha, const void *A,
+                     size_t lda, const void *x, double beta, void *y);
 /* Reference-geometry GEMV (KernelMatvecBuffer.hpp:180-325), bit-identical to
  * the reference's host check (KernelMatvecBuffer.hpp:118-133). */
 int gat_gemv_compat(gat_ctx *ctx, int dtype, int transA, size_t M, size_t N, int gemv, double alpha,
