@@ -72,30 +72,154 @@ def timed(ctx, fn, steps, warmup):
     return ms / steps, wall
 
 
-def cpu_baseline(n, budget_s):
-    """Reference buffer-kernel arithmetic (k-ascending unfused chains) on the
-    host cores, on a bounded sample of output rows of the n^3 product."""
+def _host_info():
+    """CPU model, host threads and what clinfo reports about OpenCL devices
+    (the reference's intended CPU baseline needs a CPU OpenCL device)."""
+    import shutil
+    import subprocess
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    probe = "clinfo not installed"
+    if shutil.which("clinfo"):
+        try:
+            txt = subprocess.run(["clinfo"], capture_output=True, text=True, timeout=60).stdout
+            types = [l.split("Device Type", 1)[1].strip() for l in txt.splitlines() if "Device Type" in l]
+            plats = [l for l in txt.splitlines() if l.strip().startswith("Number of platforms")]
+            probe = "%s; device types: %s; CPU OpenCL device: %s" % (
+                plats[0].strip() if plats else "no platform line", ", ".join(types) or "none",
+                "yes" if any("CPU" in t for t in types) else "no")
+        except (OSError, subprocess.SubprocessError) as e:
+            probe = "clinfo failed: %s" % e
+    return {"cpu_model": model, "host_cpus": os.cpu_count(), "clinfo": probe}
+
+
+def _uniform_into(out, seed, stream, chunk=1 << 24):
+    """The device generator's values (gat_fill_uniform) on the host, chunk by chunk."""
+    import numpy as np
+
+    from tests.helpers import uniform_host
+    for i0 in range(0, out.size, chunk):
+        i1 = min(out.size, i0 + chunk)
+        out[i0:i1] = uniform_host(0, seed, stream, idx=np.arange(i0, i1, dtype=np.uint64))
+    return out
+
+
+def _trials(fn, trials):
+    """The reference's timing: one dummy run (GatlasAppUtil.cpp:203-208),
+    then `trials` timed runs (wall clock around the compute only)."""
+    fn()
+    ts = []
+    for _ in range(trials):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return ts
+
+
+def _rates(work, ts, scale):
+    """Median and mean of the per-trial rates (GatlasAppUtil.cpp:252-264
+    averages per-trial GFLOPS)."""
+    import numpy as np
+    r = [work / t / scale for t in ts]
+    return float(np.median(r)), float(np.mean(r))
+
+
+def cpu_baseline(n, budget_s, gpu, trials=5):
+    """The reference buffer-kernel arithmetic (k-ascending unfused multiply-add
+    chains per output, KernelMatmulBuffer.hpp:259-337; the GEMV and SAXPY
+    kernels' chains likewise), ported to C with OpenMP over output rows, on
+    the host cores of the GPU box, on the GPU run's own seeded inputs.  No CPU
+    OpenCL runtime exists in this image (clinfo is recorded), so the port
+    stands in for the reference's kernels on a CPU OpenCL device.  Each leg:
+    one dummy run, then `trials` timed runs; median and mean of the per-trial
+    rates.  The headline value is SGEMM n^3 in the general form on a bounded
+    sample of output rows (about budget_s of CPU work in all)."""
     import numpy as np
 
     from oracle import oracle as ora
-    from tests.helpers import uniform_host
+    info = _host_info()
+    legs = {}
 
-    A = uniform_host(n * n, 0x6A7A5EED, 0)
-    B = uniform_host(n * n, 0x6A7A5EED, 1)
-    C = uniform_host(n * n, 0x6A7A5EED, 2)
-    rows = 8
+    inputs = {}
+
+    def gemm_leg(m, ta, tb, general, rows=None):
+        if m not in inputs:
+            inputs.clear()
+            inputs[m] = [_uniform_into(np.empty(m * m, np.float32), 0x6A7A5EED, s) for s in range(3)]
+        A, B, C = inputs[m]
+        r = m if rows is None else rows
+        kw = dict(gemm=general, alpha=1.5, beta=-0.5) if general else {}
+        ts = _trials(lambda: ora.port_matmul_rows(A, B, C, m, m, m, 0, r, transA=ta, transB=tb, **kw), trials)
+        med, mean = _rates(2.0 * r * m * m, ts, 1e12)
+        return {"TFLOP/s_median": round(med, 5), "TFLOP/s_mean": round(mean, 5), "trial_s": round(float(np.median(ts)), 4),
+                "rows": r}
+
+    # headline: rows of the n^3 general-form product, sized from an 8-row run
+    A = _uniform_into(np.empty(n * n, np.float32), 0x6A7A5EED, 0)
+    B = _uniform_into(np.empty(n * n, np.float32), 0x6A7A5EED, 1)
+    C = _uniform_into(np.empty(n * n, np.float32), 0x6A7A5EED, 2)
     t0 = time.perf_counter()
-    ora.port_matmul_rows(A, B, C, n, n, n, 0, rows, gemm=True, alpha=1.5, beta=-0.5)
-    t1 = time.perf_counter() - t0
-    rows2 = int(max(rows, min(n, rows * budget_s / max(t1, 1e-3))))
-    t0 = time.perf_counter()
-    ora.port_matmul_rows(A, B, C, n, n, n, 0, rows2, gemm=True, alpha=1.5, beta=-0.5)
-    t2 = time.perf_counter() - t0
-    tflops = 2.0 * rows2 * n * n / t2 / 1e12
-    return {"value": tflops, "unit": "TFLOP/s", "cores": ora.num_threads(), "kind": "port",
-            "sample": "%d of %d output rows of the %d^3 FP32 general-form product (%.1f s); reference "
-                      "buffer-kernel arithmetic: k-ascending unfused multiply-add chains, OpenMP over rows"
-                      % (rows2, n, n, t2)}
+    ora.port_matmul_rows(A, B, C, n, n, n, 0, 8, gemm=True, alpha=1.5, beta=-0.5)
+    per_row = (time.perf_counter() - t0) / 8
+    rows = int(max(8, min(n, budget_s / (trials + 1) / max(per_row, 1e-6))))
+    ts = _trials(lambda: ora.port_matmul_rows(A, B, C, n, n, n, 0, rows, gemm=True, alpha=1.5, beta=-0.5), trials)
+    med, mean = _rates(2.0 * rows * n * n, ts, 1e12)
+    del A, B, C
+    # the reference's own runnable case and the 1024/2048 sweep points
+    legs["sgemm_1024_pure"] = gemm_leg(1024, 0, 0, False)
+    for m in (1024, 2048):
+        for ta in (0, 1):
+            for tb in (0, 1):
+                legs["general_%d_%s%s" % (m, "T" if ta else "N", "T" if tb else "N")] = gemm_leg(m, ta, tb, True)
+    inputs.clear()
+    # SGEMV y = A x + y, 16384^2 (reference chain: one row per work-item)
+    m = 16384
+    Av = _uniform_into(np.empty(m * m, np.float32), 5, 0)
+    xv = _uniform_into(np.empty(m, np.float32), 5, 1)
+    yv = _uniform_into(np.empty(m, np.float32), 5, 2)
+    ts = _trials(lambda: ora.port_matvec_rows(Av, xv, yv, m, 0, m, gemv=True, alpha=1.0, beta=1.0), trials)
+    med_v, mean_v = _rates(4.0 * (m * m + 3 * m), ts, 1e9)
+    legs["sgemv_16384"] = {"GB/s_median": round(med_v, 2), "GB/s_mean": round(mean_v, 2),
+                           "trial_s": round(float(np.median(ts)), 4)}
+    del Av
+    # SAXPY z = 1.5 x + y, n = 2^28 (separate z, as the reference writes it)
+    ns = 1 << 28
+    X = _uniform_into(np.empty(ns, np.float32), 7, 0)
+    Y = _uniform_into(np.empty(ns, np.float32), 7, 1)
+    Z = np.empty(ns, np.float32)
+    ts = _trials(lambda: ora.port_saxpy(1.5, X, Y, Z), trials)
+    med_s, mean_s = _rates(12.0 * ns, ts, 1e9)
+    legs["saxpy_2^28"] = {"GB/s_median": round(med_s, 2), "GB/s_mean": round(mean_s, 2),
+                          "trial_s": round(float(np.median(ts)), 4)}
+    del X, Y, Z
+    # GPU over CPU on the same workloads (GPU figures from this run)
+    ratio = {}
+    if gpu.get("sgemm_1024"):
+        ratio["sgemm_1024_pure"] = round(gpu["sgemm_1024"] / legs["sgemm_1024_pure"]["TFLOP/s_median"], 1)
+    if gpu.get("sgemv_16384"):
+        ratio["sgemv_16384"] = round(gpu["sgemv_16384"] / legs["sgemv_16384"]["GB/s_median"], 1)
+    if gpu.get("saxpy"):
+        ratio["saxpy_2^28"] = round(gpu["saxpy"] / legs["saxpy_2^28"]["GB/s_median"], 1)
+    for k, v in gpu.get("sweep", {}).items():
+        leg = "general_" + k[4:]
+        if leg in legs:
+            ratio[leg] = round(v / legs[leg]["TFLOP/s_median"], 1)
+    out = {"value": round(med, 6), "unit": "TFLOP/s", "cores": ora.num_threads(), "kind": "port",
+           "sample": "%d of %d output rows of the %d^3 FP32 general-form product per trial; one dummy run, "
+                     "then %d trials (median; mean %.6f TFLOP/s); reference buffer-kernel arithmetic: "
+                     "k-ascending unfused multiply-add chains, OpenMP over rows" % (rows, n, n, trials, mean),
+           "trials": trials, "TFLOP/s_mean": round(mean, 6)}
+    out.update(info)
+    out["legs"] = legs
+    out["gpu_over_cpu"] = ratio
+    return out
 
 
 def fp64_errors(got, A, B, M, N, K, rows, transA=False, transB=False):
@@ -210,12 +334,9 @@ def grouped_bench(ctx, dtype, reps=10):
     return out
 
 
-def sgemm_1024_bench(ctx, with_cpu):
+def sgemm_1024_bench(ctx):
     """The reference's own runnable case: SGEMM C = A*B, M = N = K = 1024,
-    FP32, no transpose, on the GPU and, timed in the same run, the reference
-    buffer-kernel arithmetic ported to the host cores (whole product)."""
-    import numpy as np
-
+    FP32, no transpose (its CPU counterpart is a cpu_baseline leg)."""
     n = 1024
     A, B, C = ctx.empty(n * n), ctx.empty(n * n), ctx.empty(n * n)
     ctx.fill_uniform(A, 0x6A7A5EED, 0)
@@ -225,16 +346,6 @@ def sgemm_1024_bench(ctx, with_cpu):
            "tile(wm,wn,tm,tn,bk,blk,wk)": list(ctx.tile_for(n, n, n))}
     for a in (A, B, C):
         a.free()
-    if with_cpu:
-        from oracle import oracle as ora
-        from tests.helpers import uniform_host
-        hA, hB = uniform_host(n * n, 0x6A7A5EED, 0), uniform_host(n * n, 0x6A7A5EED, 1)
-        hC = np.zeros(n * n, np.float32)
-        t0 = time.perf_counter()
-        ora.port_matmul_rows(hA, hB, hC, n, n, n, 0, n)
-        t = time.perf_counter() - t0
-        out.update({"cpu_port_s": round(t, 4), "cpu_port_TFLOP/s": round(2.0 * n ** 3 / t / 1e12, 4),
-                    "cpu_threads": ora.num_threads(), "gpu_over_cpu": round(t / (ms * 1e-3), 1)})
     return out
 
 
@@ -266,36 +377,94 @@ def sweep_bench(ctx, sizes=(1024, 2048, 4096, 8192)):
     return out
 
 
-def tuner_bench(ctx, sizes=(1024, 2048, 4096)):
-    """The autotuner's search over the MFMA tile table per size bucket: the
-    default tile against the installed one, each timed in three alternating
-    rounds after the search (median), so clock drift cannot favour either."""
+def _journal_records(path):
+    """Timed trials a journal holds (records with a value >= 0), and keys."""
+    n, keys = 0, set()
+    with open(path) as f:
+        for line in f:
+            k, _, v = line.rstrip("\n").rpartition("\t")
+            if v.lstrip("-").isdigit() and int(v) >= 0:
+                n += 1
+                keys.add(k)
+    return n, len(keys)
+
+
+def tuner_bench(ctx, sizes=(1024, 2048, 4096, 8192)):
+    """The autotuning searches at the workloads' sizes, journaled:
+      * gat_tune_gemm (times every instantiated MFMA tile of the form, then
+        confirms the winner as the EM fixed point) for FP32 and FP64, all four
+        transpose pairs, general form, square sizes; the default tile and the
+        installed one are then timed in three alternating rounds (median);
+      * the reference's EM search (benchMatvec.cpp:521-612,
+        benchSaxpy.cpp:498-584) over the GEMV y = A*x + y at 16384^2 (both
+        storage orders) and SAXPY at n = 2^28, for the bandwidth kernels and
+        the reference-geometry buffer kernels, through the parameter
+        interface.
+    Per case: search seconds, candidates journaled, best parameters."""
     import tempfile
 
+    import numpy as np
+
+    from gatlas_amd import _capi
+    from gatlas_amd.interface import BUFFER, MATVEC, NATIVE, SAXPY, Kernel
     out = {}
     with tempfile.TemporaryDirectory() as d:
-        for n in sizes:
-            A, B, C = ctx.empty(n * n), ctx.empty(n * n), ctx.empty(n * n)
+        for dt, name, npdt in ((_capi.GAT_F32, "f32", np.float32), (_capi.GAT_F64, "f64", np.float64)):
+            nmax = max(sizes)
+            A, B, C = (ctx.empty(nmax * nmax, npdt) for _ in range(3))
             for i, a in enumerate((A, B, C)):
                 ctx.fill_uniform(a, 0x6A7A5EED, i)
-            # about 5 ms of launches at 100 TFLOP/s, 10..200 of them
-            steps = max(10, min(200, int(5e-3 * 100e12 / (2.0 * n ** 3))))
-            ctx.tuned_clear()
-            dcfg = ctx.tile_for(n, n, n)
-            t0 = time.perf_counter()
-            cfg, _ = ctx.tune_gemm(n, n, n, journal=os.path.join(d, "tune_%d.journal" % n))
-            tune_s = time.perf_counter() - t0
-            ms = {"default": [], "tuned": []}
-            for _ in range(3):
-                for key, c in (("default", dcfg), ("tuned", cfg)):
-                    ms[key].append(timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n, cfg=c), steps, 5)[0])
-            tf = {k: 2.0 * n ** 3 / (sorted(v)[1] * 1e-3) / 1e12 for k, v in ms.items()}
-            out["sgemm_%d" % n] = {"default_TFLOP/s": round(tf["default"], 2), "tuned_TFLOP/s": round(tf["tuned"], 2),
-                                   "tile_default": list(dcfg), "tile(wm,wn,tm,tn,bk,blk,wk)": list(cfg),
-                                   "search_s": round(tune_s, 2)}
+            for n in sizes:
+                steps = max(3, min(200, int(5e-3 * 100e12 / (2.0 * n ** 3))))
+                for ta in (0, 1):
+                    for tb in (0, 1):
+                        ctx.tuned_clear()
+                        dcfg = ctx.tile_for(n, n, n, bool(ta), bool(tb), dt)
+                        jp = os.path.join(d, "tune_%s_%d_%d%d.journal" % (name, n, ta, tb))
+                        t0 = time.perf_counter()
+                        cfg, _ = ctx.tune_gemm(n, n, n, bool(ta), bool(tb), dtype=dt, journal=jp)
+                        tune_s = time.perf_counter() - t0
+                        ms = {"default": [], "tuned": []}
+                        for _ in range(3):
+                            for key, c in (("default", dcfg), ("tuned", cfg)):
+                                ms[key].append(timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n, bool(ta), bool(tb), 1.5,
+                                                                           -0.5, dtype=dt, cfg=c), steps, 2)[0])
+                        tf = {k: 2.0 * n ** 3 / (sorted(v)[1] * 1e-3) / 1e12 for k, v in ms.items()}
+                        trials, keys = _journal_records(jp)
+                        out["%s_%d_%s%s" % (name, n, "T" if ta else "N", "T" if tb else "N")] = {
+                            "default_TFLOP/s": round(tf["default"], 2), "tuned_TFLOP/s": round(tf["tuned"], 2),
+                            "tile_default": list(dcfg), "tile_tuned": list(cfg), "search_s": round(tune_s, 2),
+                            "candidates_journaled": keys, "trials_journaled": trials}
             for a in (A, B, C):
                 a.free()
-    ctx.tuned_clear()
+        ctx.tuned_clear()
+        m, ns = 16384, 1 << 28
+        searches = [("sgemv_16384_native", MATVEC, NATIVE, 4, dict(M=m, N=m, generalized=True)),
+                    ("sgemv_16384_T_native", MATVEC, NATIVE, 4, dict(M=m, N=m, transA=True, generalized=True)),
+                    ("sgemv_16384_buffer_float4", MATVEC, BUFFER, 4, dict(M=m, N=m, generalized=True)),
+                    ("saxpy_2^28_native", SAXPY, NATIVE, 4, dict(M=ns // 4, N=4)),
+                    ("saxpy_2^28_buffer_float4", SAXPY, BUFFER, 4, dict(M=ns // 4, N=4))]
+        for key, fam, var, vl, kw in searches:
+            k = Kernel(fam, _capi.GAT_F32, vl, var, ctx=ctx)
+            jp = os.path.join(d, key + ".journal")
+            lim = dict(max_block_height=64, max_group_size=256) if fam == MATVEC else \
+                dict(max_block_height=16, max_group_size=256)
+            t0 = time.perf_counter()
+            best = k.em_search(jp, vl=vl, max_workgroup=ctx.max_workgroup(), **lim, **kw)
+            search_s = time.perf_counter() - t0
+            trials, keys = _journal_records(jp)
+            res = {"search_s": round(search_s, 2), "candidates_journaled": keys, "trials_journaled": trials,
+                   "best_params": best}
+            if best:
+                from gatlas_amd.interface import Journal
+                j = Journal(jp)
+                j.loadMemo()
+                us = j.memoTime(k, best, 0)
+                if us and us > 0:
+                    nbytes = 4.0 * (m * m + 3 * m) if fam == MATVEC else 12.0 * ns
+                    res["best_us"] = us
+                    res["best_GB/s"] = round(nbytes / us / 1e3, 1)
+            out[key] = res
     return out
 
 
@@ -340,18 +509,46 @@ def mixed_errors(ctx, n, nrows=32):
     return out
 
 
-def serve_bench(ctx, rate, mode, seconds, gemm_dtype=0):
+def serve_bench(ctx, rate, mode, seconds, gemm_dtype=0, keep=20):
     """Open-loop Poisson trace: 70% GEMM (grouped-workload shapes, FP32
     operands; computed in FP32 or split-BF16x3), 30% GEMV (M, N log-uniform
-    in [1024, 32768]), resident operands, 64 GB admission budget."""
+    in [1024, 32768]), resident operands, 64 GB admission budget (operands
+    count as views into the resident pool, results as arena allocations).
+    About `keep` results spread over the trace are kept: each is recomputed
+    alone (bit-identical required) and a row sample of each is compared with
+    the FP64 product (numpy) of its operands."""
+    import numpy as np
+
     from gatlas_amd.device import Server
-    srv = Server(ctx, rate=float(rate), duration=float(seconds), mode=mode, seed=rate, gemm_dtype=gemm_dtype)
+    nreq = rate * seconds
+    srv = Server(ctx, rate=float(rate), duration=float(seconds), mode=mode, seed=rate, gemm_dtype=gemm_dtype,
+                 keep=max(1, int(nreq // keep)), keep_max=keep)
     st = srv.run()
+    mism = srv.check_alone()
+    worst = 0.0
+    for r in srv.kept():
+        M, N, K = r["M"], r["N"], r["K"]
+        rows = np.unique(np.linspace(0, M - 1, min(M, 4)).astype(np.int64))
+        if r["kind"] == 0:
+            A, B = srv.read(r["A"], M * K), srv.read(r["B"], K * N)
+            got = srv.read(r["C"], M * N).reshape(M, N)[rows]
+            cw, _ = fp64_errors(got, A, B, M, N, K, rows, bool(r["transA"]), bool(r["transB"]))
+        else:
+            A = np.concatenate([srv.read(r["A"] + 4 * int(i) * r["lda"], N) for i in rows])
+            x = srv.read(r["B"], N)
+            got = srv.read(r["C"], M)[rows]
+            cw, _ = fp64_errors(got[:, None], A, x, len(rows), 1, N, np.arange(len(rows)))
+        worst = max(worst, cw)
     srv.close()
-    keys = ("requests", "launches", "wall_s", "tflops", "req_per_s", "p50_ms", "p99_ms", "p50_gemm_ms",
-            "p99_gemm_ms", "p50_gemv_ms", "p99_gemv_ms", "mean_queue_ms", "mean_batch")
+    keys = ("requests", "launches", "wall_s", "tflops", "offered_tflops", "req_per_s", "p50_ms", "p99_ms",
+            "p50_gemm_ms", "p99_gemm_ms", "p50_gemv_ms", "p99_gemv_ms", "mean_queue_ms", "mean_batch",
+            "device_busy", "blocked_ms")
     out = {k: (round(st[k], 4) if isinstance(st[k], float) else st[k]) for k in keys}
     out["peak_footprint_GB"] = round(st["peak_footprint"] / 1e9, 2)
+    out["peak_output_GB"] = round(st["peak_output_bytes"] / 1e9, 3)
+    out["kept_checked"] = st["kept"]
+    out["kept_alone_mismatches"] = mism
+    out["kept_max_componentwise_err"] = float("%.3e" % worst)
     return out
 
 
@@ -471,7 +668,7 @@ def main():
         sec["dgemm_8192"] = {"ms": round(ms_d, 3), "TFLOP/s": round(tf, 2), "frac_fp64": round(tf / PEAK_FP64, 4)}
         for a in (Ad, Bd, Cd):
             a.free()
-        sec["sgemm_1024"] = sgemm_1024_bench(ctx, not args.no_cpu)
+        sec["sgemm_1024"] = sgemm_1024_bench(ctx)
         sec["general_form_sweep"] = sweep_bench(ctx)
         sec["tuner"] = tuner_bench(ctx)
     if not args.no_grouped:
@@ -487,7 +684,12 @@ def main():
     ctx.close()
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
+        gpu = {"sgemm_1024": sec.get("sgemm_1024", {}).get("gpu_TFLOP/s"),
+               "sgemv_16384": sec.get("sgemv_16384", {}).get("GB/s"),
+               "saxpy": sec.get("saxpy_2^28", {}).get("GB/s"),
+               "sweep": {k: v for k, v in sec.get("general_form_sweep", {}).items()
+                         if k.startswith("f32_1024_") or k.startswith("f32_2048_")}}
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds, gpu)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -58,7 +58,8 @@ class ServeStats(ctypes.Structure):
     _fields_ = [(n, _c_sz) for n in ("requests", "gemm_requests", "gemv_requests", "launches", "kept")] + \
                [(n, _c_dbl) for n in ("wall_s", "flops", "bytes", "tflops", "req_per_s", "p50_ms", "p99_ms",
                                       "max_ms", "p50_gemm_ms", "p99_gemm_ms", "p50_gemv_ms", "p99_gemv_ms",
-                                      "mean_queue_ms", "mean_batch", "peak_footprint")]
+                                      "mean_queue_ms", "mean_batch", "peak_footprint", "device_busy", "blocked_ms",
+                                      "offered_tflops", "peak_output_bytes")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -66,7 +66,7 @@ void gat_serve_cfg_default(gat_serve_cfg* cfg) {
     cfg->rate = 2000.0;
     cfg->duration = 1.0;
     cfg->gemm_fraction = 0.7;
-    cfg->hbm_budget = (size_t)64 << 30;
+    cfg->hbm_budget = (size_t)64000000000ull;  /* 64 GB */
     cfg->mode = GAT_SERVE_BATCHED;
     cfg->streams = 4;
     cfg->max_batch = 256;

@@ -365,12 +365,19 @@ int Server::run(gat_serve_stats* out) {
     std::deque<int> queue;
     std::vector<int> pend[2];
     std::vector<std::deque<Launch>> inflight(streams_.size());
-    size_t admitted_bytes = 0, peak = 0, launches = 0, launched_reqs = 0;
-    int admitted = 0, done = 0, next = 0, rr = 0;
+    size_t admitted_bytes = 0, peak = 0, launches = 0, launched_reqs = 0, peak_out = 0;
+    int admitted = 0, done = 0, next = 0, rr = 0, running = 0;
+    // attribution: time with a launch in flight (host-observed, from issue
+    // to observed completion) and time requests waited at a stopped admission
+    double busy = 0.0, blocked = 0.0, tlast = 0.0;
+    bool was_running = false, was_blocked = false;
     const bool batched = cfg_.mode == GAT_SERVE_BATCHED;
     const double t0 = now_s();
     while (done < n) {
         const double t = now_s() - t0;
+        if (was_running) busy += t - tlast;
+        if (was_blocked) blocked += t - tlast;
+        tlast = t;
         bool progress = false;
         while (next < n && reqs_[next].t_arr <= t) queue.push_back(next++);
         // completions: launches on one stream finish in order
@@ -393,6 +400,7 @@ int Server::run(gat_serve_stats* out) {
                 events_free_.push_back(L.ev);
                 if (L.slot >= 0) slots_[L.slot].busy = false;
                 inflight[s].pop_front();
+                --running;
                 progress = true;
             }
         }
@@ -406,11 +414,15 @@ int Server::run(gat_serve_stats* out) {
             r.t_adm = now_s() - t0;
             admitted_bytes += r.foot;
             peak = std::max(peak, admitted_bytes);
+            peak_out = std::max(peak_out, arena_.in_use());
             ++admitted;
             pend[r.kind].push_back(queue.front());
             queue.pop_front();
             progress = true;
         }
+        // arrived requests left in the queue wait on the budget, max_inflight
+        // or the arena
+        was_blocked = !queue.empty();
         if (!queue.empty() && admitted == 0) {
             // nothing in flight will free memory: kept results fill the arena
             set_error("serve: a request cannot be admitted (kept results hold the output arena)");
@@ -431,6 +443,7 @@ int Server::run(gat_serve_stats* out) {
                     if (rc) return rc;
                     pend[kind].clear();
                     inflight[sb].push_back(std::move(L));
+                    ++running;
                     ++launches;
                     ++launched_reqs;
                     progress = true;
@@ -451,6 +464,7 @@ int Server::run(gat_serve_stats* out) {
                 if (rc) return rc;
                 slots_[s].busy = true;
                 launched_reqs += L.members.size();
+                ++running;
                 ++launches;
                 inflight[L.stream].push_back(std::move(L));
                 progress = true;
@@ -463,6 +477,7 @@ int Server::run(gat_serve_stats* out) {
                     if (rc) return rc;
                     inflight[rr].push_back(std::move(L));
                     rr = (rr + 1) % (int)streams_.size();
+                    ++running;
                     ++launches;
                     ++launched_reqs;
                 }
@@ -470,6 +485,7 @@ int Server::run(gat_serve_stats* out) {
             }
         }
         (void)progress;
+        was_running = running > 0;
     }
     for (hipStream_t s : streams_) GAT_HIP_TRY(hipStreamSynchronize(s));
     if (out) {
@@ -503,6 +519,10 @@ int Server::run(gat_serve_stats* out) {
         out->mean_queue_ms = n ? qsum / n : 0.0;
         out->mean_batch = launches ? (double)launched_reqs / launches : 0.0;
         out->peak_footprint = (double)peak;
+        out->device_busy = out->wall_s > 0 ? std::min(1.0, busy / out->wall_s) : 0.0;
+        out->blocked_ms = blocked * 1e3;
+        out->offered_tflops = out->flops / cfg_.duration * 1e-12;
+        out->peak_output_bytes = (double)peak_out;
     }
     return ST_OK;
 }

@@ -257,7 +257,11 @@ typedef struct gat_serve_cfg {
     double rate;                  /* mean arrivals per second (Poisson)            */
     double duration;              /* seconds of arrivals                           */
     double gemm_fraction;         /* share of GEMM requests; the rest are GEMV     */
-    size_t hbm_budget;            /* bytes of admitted footprint at once           */
+    size_t hbm_budget;            /* bytes of admitted footprint at once: every    */
+                                  /* request counts its operands and its result;   */
+                                  /* operands are views into one resident pool, so */
+                                  /* their share is a working-set bound, results   */
+                                  /* are allocated in an arena of this size        */
     int mode;                     /* GAT_SERVE_BATCHED or GAT_SERVE_LOOP           */
     int streams;                  /* device streams launches overlap on            */
     int max_batch;                /* problems per batched launch                   */
@@ -280,6 +284,12 @@ typedef struct gat_serve_stats {
     double mean_queue_ms;         /* arrival to admission                          */
     double mean_batch;            /* requests per launch                           */
     double peak_footprint;        /* bytes admitted at once, maximum               */
+    double device_busy;           /* share of wall time with a launch in flight    */
+    double blocked_ms;            /* time requests waited while admission was     */
+                                  /* stopped by the budget, max_inflight or arena */
+    double offered_tflops;        /* the trace's flops / its arrival duration      */
+    double peak_output_bytes;     /* output-arena bytes held at once (the part of  */
+                                  /* the footprint that is a real allocation)      */
 } gat_serve_stats;
 
 typedef struct gat_serve_req {

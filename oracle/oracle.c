@@ -402,83 +402,6 @@ void ora_error_metrics_f64(const double *got, const double *ref, const double *m
     out[2] = amax;
 }
 
-/* ------------------------------------------------------------------ */
-/* CPU port of the reference buffer-kernel arithmetic (timed baseline) */
-/* ------------------------------------------------------------------ */
-
-/* C rows [row0, row1) of C = alpha*op(A)op(B) + beta*C (gemm) or
- * C = op(A)op(B): every element is a k-ascending chain of unfused
- * multiply-then-add in float, the arithmetic of KernelMatmulBuffer.hpp:259-337
- * with its host check's rounding.  Loops run i, k, j so that the j loop
- * vectorises across independent output elements without changing any
- * element's k order.  Returns nothing; the caller times it. */
-void ora_port_matmul_f32_rows(int gemm, int transA, int transB,
-                              size_t M, size_t N, size_t K,
-                              float alpha, float beta,
-                              const float *A, const float *B, float *C,
-                              size_t row0, size_t row1)
-{
-    const size_t JB = 512;
-    _Pragma("omp parallel")
-    {
-        float acc[512];
-        float bcol[512];
-        _Pragma("omp for schedule(dynamic, 1) collapse(2)")
-        for (size_t i = row0; i < row1; i++)
-            for (size_t j0 = 0; j0 < N; j0 += JB) {
-                const size_t jn = (j0 + JB < N) ? JB : N - j0;
-                for (size_t j = 0; j < jn; j++) acc[j] = 0.0f;
-                for (size_t k = 0; k < K; k++) {
-                    const float a = transA ? A[k * M + i] : A[i * K + k];
-                    const float *brow;
-                    if (!transB) {
-                        brow = B + k * N + j0;
-                    } else {
-                        for (size_t j = 0; j < jn; j++) bcol[j] = B[(j0 + j) * K + k];
-                        brow = bcol;
-                    }
-                    for (size_t j = 0; j < jn; j++) {
-                        const float prod = a * brow[j];
-                        acc[j] = acc[j] + prod;
-                    }
-                }
-                float *crow = C + i * N + j0;
-                if (gemm) {
-                    for (size_t j = 0; j < jn; j++) {
-                        const float x = alpha * acc[j];
-                        const float y = beta * crow[j];
-                        crow[j] = x + y;
-                    }
-                } else {
-                    for (size_t j = 0; j < jn; j++) crow[j] = acc[j];
-                }
-            }
-    }
-}
-
-/* y = alpha*A*x + beta*y (gemv) over rows [row0,row1), j-ascending unfused. */
-void ora_port_matvec_f32_rows(int gemv, size_t N, float alpha, float beta,
-                              const float *A, const float *x, float *y,
-                              size_t row0, size_t row1)
-{
-    _Pragma("omp parallel for schedule(static)")
-    for (size_t i = row0; i < row1; i++) {
-        float acc = 0.0f;
-        const float *arow = A + i * N;
-        for (size_t j = 0; j < N; j++) {
-            const float prod = arow[j] * x[j];
-            acc = acc + prod;
-        }
-        if (gemv) {
-            const float u = alpha * acc;
-            const float v = beta * y[i];
-            y[i] = u + v;
-        } else {
-            y[i] = acc;
-        }
-    }
-}
-
 int ora_num_threads(void)
 {
 #ifdef _OPENMP

@@ -69,6 +69,7 @@ def lib():
                                                [_f32p, _f32p, _f32p, _sz, _sz])
         L.ora_port_matvec_f32_rows.argtypes = [ctypes.c_int, _sz, ctypes.c_float, ctypes.c_float,
                                                _f32p, _f32p, _f32p, _sz, _sz]
+        L.ora_port_saxpy_f32.argtypes = [_sz, _sz, ctypes.c_float, _f32p, _f32p, _f32p]
         L.ora_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -211,6 +212,11 @@ def port_matmul_rows(A, B, C, M, N, K, row0, row1, gemm=False, alpha=1.0, beta=0
 def port_matvec_rows(A, x, y, N, row0, row1, gemv=False, alpha=1.0, beta=0.0):
     lib().ora_port_matvec_f32_rows(int(gemv), N, np.float32(alpha), np.float32(beta),
                                    A.ravel(), x, y, row0, row1)
+
+
+def port_saxpy(alpha, x, y, z, i0=0, i1=None):
+    """CPU port of the reference SAXPY arithmetic into a preallocated z."""
+    lib().ora_port_saxpy_f32(i0, x.size if i1 is None else i1, np.float32(alpha), x, y, z)
 
 
 def num_threads():

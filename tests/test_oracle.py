@@ -250,6 +250,38 @@ def test_port_equals_reference_chain_bitwise():
     assert np.array_equal(y, ora.ref_matvec(A, x, M, N))
 
 
+def test_vectorised_port_keeps_every_elements_rounding():
+    # The port is built -O3 (vectorised across output elements, per-ISA
+    # clones): ragged widths, general forms and the SAXPY port still give the
+    # reference chain's bits, where a fused multiply-add or a reassociated
+    # sum would not.
+    rng = np.random.default_rng(11)
+    M, N, K = 37, 1029, 211
+    A = (rng.random(M * K, dtype=np.float32) * 2 - 1)
+    B = (rng.random(K * N, dtype=np.float32) * 2 - 1)
+    Cold = (rng.random(M * N, dtype=np.float32) * 2 - 1)
+    want = ora.ref_matmul(A, B, M, N, K, gemm=True, alpha=1.5, beta=-0.5, Cold=Cold)
+    C = Cold.copy()
+    ora.port_matmul_rows(A, B, C, M, N, K, 0, M, gemm=True, alpha=1.5, beta=-0.5)
+    assert np.array_equal(C.view(np.uint32), want.view(np.uint32))
+    Av = (rng.random(M * N, dtype=np.float32) * 2 - 1)
+    x = (rng.random(N, dtype=np.float32) * 2 - 1)
+    y0 = (rng.random(M, dtype=np.float32) * 2 - 1)
+    y = y0.copy()
+    ora.port_matvec_rows(Av, x, y, N, 0, M, gemv=True, alpha=0.7109375, beta=-1.3125)
+    ref = ora.ref_matvec(Av, x, M, N, gemv=True, alpha=0.7109375, beta=-1.3125, yold=y0)
+    assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
+    n = 100003
+    xs = (rng.random(n, dtype=np.float32) * 2 - 1)
+    ys = (rng.random(n, dtype=np.float32) * 2 - 1)
+    z = np.empty(n, np.float32)
+    ora.port_saxpy(0.7109375, xs, ys, z)
+    want = ora.ref_saxpy(np.float32(0.7109375), xs, ys)
+    assert np.array_equal(z.view(np.uint32), want.view(np.uint32))
+    fused = (np.float64(0.7109375) * xs.astype(np.float64) + ys).astype(np.float32)
+    assert not np.array_equal(fused, want)   # the check can tell one rounding from two
+
+
 def test_planted_mistake_is_caught():
     # A transposed-operand slip planted in a copy of the oracle's result must
     # fail the integer-exact check used above.
