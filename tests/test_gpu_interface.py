@@ -7,10 +7,13 @@ command-line tools and the autotuner."""
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 from gatlas_amd import _capi
 from gatlas_amd.interface import BUFFER, IMAGE, MATMUL, MATVEC, MFMA, NATIVE, SAXPY, Journal, Kernel
+from oracle import oracle as ora
+from tests.helpers import uniform_host
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -173,3 +176,22 @@ def test_transfer_inclusive_timing(tmp_path):
     both = _point_usec(tmp_path, "sr.jnl", ["-s", "-r"])
     print("[timing] device-only %.0f us, with transfers %.0f us" % (plain, both), flush=True)
     assert both > plain + 500
+
+
+def test_public_abi_cpp_example(tmp_path):
+    """The integration example (plain g++ against include/gatlas.h only)
+    runs the reference's buffer SGEMM trial, whose exact test-pattern check
+    passes, and a direct gat_gemm whose result matches the FP64 oracle."""
+    exe = os.path.join(BIN, "integration_example")
+    out_path = str(tmp_path / "C.bin")
+    rc, out, err = _run([exe, out_path])
+    assert rc == 0, (out, err)
+    assert "matmulbufferfloat4 check 1" in out and "gemm 0" in out, out
+    M, N, K = 256, 192, 320
+    A = uniform_host(M * K, 0x6A7A5EED, 0)
+    B = uniform_host(K * N, 0x6A7A5EED, 1)
+    C0 = uniform_host(M * N, 0x6A7A5EED, 2)
+    got = np.fromfile(out_path, dtype=np.float32)
+    C64, MAG = ora.gemm64(A, B, M, N, K, alpha=1.5, beta=-0.5, C0=C0)
+    cw, _, _ = ora.error_metrics(got, C64, MAG)
+    assert cw <= 1e-5, cw
