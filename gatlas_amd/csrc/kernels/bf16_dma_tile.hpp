@@ -30,6 +30,11 @@ typedef __attribute__((address_space(1))) void glb_void_t;
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) s16x4 lds_s16x4_t;
 
+// s_waitcnt operand for gfx9-family targets: vmcnt in bits 3:0 and 15:14,
+// expcnt (7: no wait) in 6:4, lgkmcnt in 11:8.  Issued as a builtin, the
+// wait is known to the compiler, which then adds none of its own after it.
+constexpr int waitcnt_imm(int vm, int lgkm) { return (vm & 15) | ((vm >> 4) << 14) | (7 << 4) | (lgkm << 8); }
+
 // One operand of a K stage: ROWS rows of 128 bytes of k (64 bf16 or 128
 // fp8 values), loaded by NW wavefronts; T is the element type.  The
 // row-contiguous layout exists for 2-byte elements only.
@@ -127,7 +132,9 @@ struct Bf16DmaShape {
 
 // Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C for
 // a tile wholly inside M x N, K >= 64 a multiple of 8, 16-byte aligned rows;
-// all WM*WN*64 threads of the group call it; ends after a barrier.  When K
+// all WM*WN*64 threads of the group call it; each wave returns with none
+// of its own loads or LDS reads in flight (a caller that reuses the LDS
+// meets the other waves at a barrier first).  When K
 // is not a multiple of 64 the first stage holds the K % 64 leading k values
 // (staged through registers, zero-padded) and every later stage is whole.
 template <int WM, int WN, int TM, int TN, bool TA, bool TB>
@@ -180,30 +187,39 @@ __device__ __forceinline__ void bf16_tile_dma(const __bf16* __restrict__ A, cons
         }
     };
 
+    constexpr int LOADS = OpA::PER_WAVE + OpB::PER_WAVE;  // loads per wave per stage
+    auto issue = [&](int st, char* buf) {
+        OpA::issue(A, lda, m0, kbase(st), buf, w, lane);
+        OpB::issue(B, ldb, n0, kbase(st), buf + OpA::BYTES, w, lane);
+    };
     if (kh) {
         OpA::stage_head(A, lda, m0, 0, kh, sb, tid, NW * 64);
         OpB::stage_head(B, ldb, n0, 0, kh, sb + OpA::BYTES, tid, NW * 64);
     } else {
-        OpA::issue(A, lda, m0, 0, sb, w, lane);
-        OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
+        issue(0, sb);
     }
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
+    if (KT > 1) {
+        issue(1, sb + STAGE);
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(LOADS, 0));  // stage 0 in LDS
+    } else {
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));
+    }
+    __builtin_amdgcn_s_barrier();
     frags(sb, 0, a0, b0);
 
     // Software pipeline over the four 16-deep steps of a stage: the
-    // fragments of step s+1 are read while the MFMAs of step s run; the
-    // wait for the next stage and the barrier come before the last step,
-    // whose MFMAs run under the reads of the next stage's first fragments.
-    // The next stage is fetched unconditionally (the last iteration fetches
-    // its own stage again into the idle buffer), so each half of the
-    // iteration is one scheduling region.
+    // fragments of step s+1 are read while the MFMAs of step s run.  Before
+    // the last step each wave waits for the next stage and for its own reads
+    // of this one, and the waves meet at a barrier; this stage's buffer is
+    // then free, and the loads of the stage after next go into it while the
+    // last step's MFMAs run under the reads of the next stage's first
+    // fragments.  A stage's loads thus have a whole iteration of matrix work
+    // to land.  Past the last stage the loads repeat it (unused), so every
+    // iteration is the same scheduling regions.
     for (int t = 0; t < KT; ++t) {
-        const char* cur = sb + (t & 1) * STAGE;
-        char* nxt = sb + ((t + 1) & 1) * STAGE;
-        const int kn = kbase(t + 1 < KT ? t + 1 : t);  // K >= 64: the last stage is whole
-        OpA::issue(A, lda, m0, kn, nxt, w, lane);
-        OpB::issue(B, ldb, n0, kn, nxt + OpA::BYTES, w, lane);
+        char* cur = sb + (t & 1) * STAGE;
+        const char* nxt = sb + ((t + 1) & 1) * STAGE;
+        __builtin_amdgcn_sched_barrier(0);
         frags(cur, 16, a1, b1);
         mma(a0, b0);
         interleave();
@@ -213,12 +229,17 @@ __device__ __forceinline__ void bf16_tile_dma(const __bf16* __restrict__ A, cons
         frags(cur, 48, a1, b1);
         mma(a0, b0);
         interleave();
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
-        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the next stage in; this wave's reads of this one done
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        issue(t + 2 < KT ? t + 2 : KT - 1, cur);
         frags(nxt, 0, a0, b0);
         mma(a1, b1);
         interleave();
+        __builtin_amdgcn_sched_barrier(0);
     }
+    __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the repeated loads land before LDS is reused
 
 #pragma unroll
     for (int i = 0; i < TM; ++i)

@@ -53,9 +53,15 @@ constexpr int fp8_waves_per_simd() {
 }
 
 // The BM x BN tile at (m0, n0), wholly inside M x N, K a multiple of 128:
-// direct-to-LDS stages, fragments of the next 64-deep step read under the
-// MFMAs of the current one, the stage wait and barrier placed before the
-// last step (see bf16_tile_dma).  arb / brb: this wave's scale-block rows.
+// 128-deep stages staged straight into two LDS buffers.  Iteration t runs
+// the MFMAs of stage t's first 64-deep half on fragments read earlier while
+// reading the second half's; waits for stage t + 1 and for its own reads,
+// meets the other waves at a barrier (after which stage t's buffer is free),
+// issues the loads of stage t + 2 into that buffer, and runs the second
+// half's MFMAs while reading the first fragments of stage t + 1.  A stage's
+// loads thus have a whole iteration of matrix work to land.  The MFMA
+// sequence is the register-staged kernel's (bit-identical results).
+// arb / brb: this wave's scale-block rows.
 template <int WM, int WN, int TM, int TN>
 __device__ __forceinline__ void fp8_tile_dma(const u8* __restrict__ A, const u8* __restrict__ B,
                                              const float* __restrict__ sA, const float* __restrict__ sB,
@@ -65,6 +71,8 @@ __device__ __forceinline__ void fp8_tile_dma(const u8* __restrict__ A, const u8*
     typedef DmaOperand<BM, true, NW, u8> OpA;
     typedef DmaOperand<BN, true, NW, u8> OpB;
     constexpr int STAGE = OpA::BYTES + OpB::BYTES;
+    constexpr int LOADS = OpA::PER_WAVE + OpB::PER_WAVE;  // loads per wave per stage
+    constexpr int NR = 2 * (TM + TN);                       // LDS reads per 64-deep half
     char* const sb = (char*)smem;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -106,59 +114,74 @@ __device__ __forceinline__ void fp8_tile_dma(const u8* __restrict__ A, const u8*
                 acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[i], b[j], acc[i][j], 0, 0, 0, sa, 0,
                                                                            sbb);
     };
-    auto interleave = [&]() {
+    // the LDS reads (and a stage's loads) spread over the first MFMAs of a
+    // half, so the reads land well before the next half needs them
+    auto interleave = [&](bool loads) {
+        constexpr int M = TM * TN, RPM = (NR + M / 2 - 1) / (M / 2), LPM = (LOADS + M / 2 - 1) / (M / 2);
 #pragma unroll
-        for (int i = 0; i < TM * TN; ++i) {
+        for (int i = 0; i < M; ++i) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // LDS reads
-            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load (to LDS)
+            if (i < M / 2) {
+                __builtin_amdgcn_sched_group_barrier(0x100, RPM, 0);  // LDS reads
+                if (loads) __builtin_amdgcn_sched_group_barrier(0x020, LPM, 0);  // global loads into LDS
+            }
         }
     };
+    auto issue = [&](int t, char* buf) {
+        OpA::issue(A, K, m0, t * F8_BK, buf, w, lane);
+        OpB::issue(B, K, n0, t * F8_BK, buf + OpA::BYTES, w, lane);
+    };
+    // Scales: lane l holds those of K tiles 64q + l (chunk q) in a vector
+    // register and each iteration takes its own with readlane.  (A scalar
+    // load shares its wait counter with the LDS reads and returns out of
+    // order, so its use would wait for every read in flight.)
     const float* sAr = sA + (int64_t)arb * KT;
     const float* sBr = sB + (int64_t)brb * KT;
+    float sav, sbv;
+    auto load_scales = [&](int t0) {
+        const int kb = t0 + lane;
+        sav = kb < KT ? gload<float>(sAr + kb) : 1.0f;
+        sbv = kb < KT ? gload<float>(sBr + kb) : 1.0f;
+    };
 
-    OpA::issue(A, K, m0, 0, sb, w, lane);
-    OpB::issue(B, K, n0, 0, sb + OpA::BYTES, w, lane);
-    int sa_c = e8m0_of(sAr[0]), sb_c = e8m0_of(sBr[0]);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    frags(sb, 0, a0, b0);
-
-    // Two fragment sets in flight need 2*(TM+TN)*8 registers beside the
-    // accumulators; where that overflows the 256 of a two-wave SIMD, each
-    // step reads its own fragments and the SIMD's other wave covers the LDS
-    // latency.  The 256 x 256 tile (4 x 2 blocks per wave, 224 registers of
-    // accumulators and fragments) fits with 212 VGPRs in all and runs 2009
-    // -> 2081 TFLOP/s pipelined at 8192^3.
-    constexpr bool kPipelined = TM * TN * 16 + 2 * (TM + TN) * 8 <= 240;
-    for (int t = 0; t < KT; ++t) {
-        const char* cur = sb + (t & 1) * STAGE;
-        char* nxt = sb + ((t + 1) & 1) * STAGE;
-        const int tn = t + 1 < KT ? t + 1 : t;  // the last iteration fetches its own stage again
-        OpA::issue(A, K, m0, tn * F8_BK, nxt, w, lane);
-        OpB::issue(B, K, n0, tn * F8_BK, nxt + OpA::BYTES, w, lane);
-        const int sa_n = e8m0_of(sAr[tn]), sb_n = e8m0_of(sBr[tn]);
-        if constexpr (kPipelined) {
-            frags(cur, 64, a1, b1);
-            mma(a0, b0, sa_c, sb_c);
-            interleave();
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
-            __syncthreads();
-            frags(nxt, 0, a0, b0);
-            mma(a1, b1, sa_c, sb_c);
-            interleave();
-        } else {
-            __builtin_amdgcn_iglp_opt(0);
-            mma(a0, b0, sa_c, sb_c);
-            frags(cur, 64, a0, b0);
-            mma(a0, b0, sa_c, sb_c);
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
-            __syncthreads();
-            frags(nxt, 0, a0, b0);
-        }
-        sa_c = sa_n;
-        sb_c = sb_n;
+    load_scales(0);
+    issue(0, sb);
+    if (KT > 1) {
+        issue(1, sb + STAGE);
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(LOADS, 0));  // the scales and stage 0 in
+    } else {
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));
     }
+    __builtin_amdgcn_s_barrier();
+    frags(sb, 0, a0, b0);
+    for (int t0 = 0; t0 < KT; t0 += 64) {
+        if (t0 > 0) {
+            load_scales(t0);
+            __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 15));
+        }
+        const int t1 = t0 + 64 < KT ? t0 + 64 : KT;
+        for (int t = t0; t < t1; ++t) {
+            char* cur = sb + (t & 1) * STAGE;
+            const int sa = e8m0_of(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(sav), t & 63)));
+            const int sbb = e8m0_of(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(sbv), t & 63)));
+            __builtin_amdgcn_sched_barrier(0);
+            frags(cur, 64, a1, b1);
+            mma(a0, b0, sa, sbb);
+            interleave(false);
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // stage t + 1 in; this wave's reads of stage t done
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_sched_barrier(0);
+            // past the last stage the loads repeat it and the reads take
+            // what is there, unused; so every iteration has the same shape
+            issue(t + 2 < KT ? t + 2 : KT - 1, cur);
+            frags(sb + ((t + 1) & 1) * STAGE, 0, a0, b0);
+            mma(a1, b1, sa, sbb);
+            interleave(true);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the repeated loads land before the wave ends
 
 #pragma unroll
     for (int i = 0; i < TM; ++i)

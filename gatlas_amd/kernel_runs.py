@@ -16,6 +16,10 @@ Workloads (inputs U[-1,1] from the device generator):
   grouped512   512 FP32 problems (M, N, K log-uniform in [64, 4096]) in one
                persistent grouped launch (untuned tile rule)
   grouped512x3 the same problems in split-BF16x3
+
+--tile W_M,W_N,T_M,T_N,BK,BLK,WK installs that tile for the GEMM workloads'
+size bucket first (the tuned-table entry the autotuner would write), so one
+tile of the table can be profiled at the workload's size.
 """
 import argparse
 import sys
@@ -31,6 +35,27 @@ def _sq(ctx, n, dt=np.float32, count=3):
     for i, a in enumerate(arrs):
         ctx.fill_uniform(a, 0x6A7A5EED, i)
     return arrs
+
+
+# GEMM workloads: (dtype, transA, transB, M, N, K) of their tuned-table bucket
+# (the FP8 product reads B as N x K, i.e. op(B) transposed)
+_GEMM_SHAPES = {
+    "sgemm8192": (_capi.GAT_F32, 0, 0, 8192, 8192, 8192),
+    "sgemm1024": (_capi.GAT_F32, 0, 0, 1024, 1024, 1024),
+    "dgemm8192": (_capi.GAT_F64, 0, 0, 8192, 8192, 8192),
+    "bf16x3_8192": (_capi.GAT_BF16X3, 0, 0, 8192, 8192, 8192),
+    "bf16_8192": (_capi.GAT_BF16, 0, 0, 8192, 8192, 8192),
+    "fp8_8192": (_capi.GAT_FP8, 0, 1, 8192, 8192, 8192),
+}
+
+
+def install_tile(ctx, name, tile):
+    import ctypes
+    if name not in _GEMM_SHAPES:
+        raise SystemExit("--tile applies to the GEMM workloads only, not %s" % name)
+    dt, ta, tb, M, N, K = _GEMM_SHAPES[name]
+    _capi.check(ctx.lib.gat_tuned_set(ctx.h, dt, ta, tb, M, N, K, ctypes.byref(_capi.TileCfg(*tile))),
+                "gat_tuned_set")
 
 
 def run(ctx, name, reps):
@@ -107,10 +132,16 @@ def run(ctx, name, reps):
 def main(argv=None):
     p = argparse.ArgumentParser()
     p.add_argument("--reps", type=int, default=3)
+    p.add_argument("--tile", default=None, help="W_M,W_N,T_M,T_N,BK,BLK,WK for the GEMM workloads")
     p.add_argument("workloads", nargs="*", help="workload names")
     a = p.parse_args(argv)
+    tile = None if a.tile is None else tuple(int(v) for v in a.tile.split(","))
+    if tile is not None and len(tile) != 7:
+        raise SystemExit("--tile takes 7 integers")
     ctx = Context(0)
     for w in a.workloads:
+        if tile is not None:
+            install_tile(ctx, w, tile)
         run(ctx, w, a.reps)
     ctx.close()
     return 0
