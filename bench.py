@@ -72,6 +72,27 @@ def timed(ctx, fn, steps, warmup):
     return ms / steps, wall
 
 
+MEDIAN_NOTE = "median of 5 rounds of 10 calls, after 3 untimed calls"
+
+
+def timed_median(ctx, fn, steps, rounds=5, warmup=3):
+    """Median over `rounds` of the per-call device time of `steps` calls in a
+    row (events on the library stream), after `warmup` untimed calls: the
+    secondary GEMMs' figure, steadier than one round against the chip's
+    clock changes under dense matrix work."""
+    for _ in range(warmup):
+        fn()
+    ctx.sync()
+    per = []
+    for _ in range(rounds):
+        ctx.timer_start()
+        for _ in range(steps):
+            fn()
+        per.append(ctx.timer_stop() / steps)
+    ctx.sync()
+    return sorted(per)[len(per) // 2]
+
+
 def _host_info():
     """CPU model, host threads and what clinfo reports about OpenCL devices
     (the reference's intended CPU baseline needs a CPU OpenCL device)."""
@@ -293,12 +314,19 @@ def grouped_bench(ctx, dtype, reps=10):
         ctx.timer_start()
         per_problem()
         loop.append(ctx.timer_stop())
-    t0 = time.perf_counter()
-    nb = ctx.grouped_tune(probs, dtype)
-    tune_s = time.perf_counter() - t0
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        jnl = os.path.join(d, "grouped.jnl")
+        t0 = time.perf_counter()
+        nb = ctx.grouped_tune(probs, dtype, journal=jnl)
+        tune_s = time.perf_counter() - t0
+        with open(jnl) as f:
+            recs = [ln.split() for ln in f if ln.strip()]
+    trials = sum(1 for r in recs if int(r[1]) >= 0)
     g = ctx.grouped(probs, dtype, use_graph=True)
     g.run()
     ctx.sync()
+    shapes = g.shapes()
     gl = []
     for _ in range(reps):
         ctx.timer_start()
@@ -323,7 +351,9 @@ def grouped_bench(ctx, dtype, reps=10):
     g50, g99 = _pcts(gl)
     l50, l99 = _pcts(loop)
     out = {"problems": 512, "gflop": round(flops / 1e9, 2), "tiles": info["tiles"], "grid": info["grid"],
-           "buckets_tuned": nb, "tune_s": round(tune_s, 2),
+           "buckets_tuned": nb, "tune_s": round(tune_s, 2), "trials_journaled": trials,
+           "interior_shapes": {name: shapes.count(i) for i, name in enumerate(("128x128", "64x64", "128x64",
+                                                                                 "64x128"))},
            "grouped_TFLOP/s": round(flops / (g50 * 1e-3) / 1e12, 2), "grouped_ms_p50": round(g50, 3),
            "grouped_ms_p99": round(g99, 3),
            "loop_TFLOP/s": round(flops / (l50 * 1e-3) / 1e12, 2), "loop_ms_p50": round(l50, 3),
@@ -603,19 +633,20 @@ def main():
     sec = _Progress()
     if not args.no_secondary:
         # mixed precision on the same operands (FP32 inputs resident in HBM)
-        ms_x3, _ = timed(ctx, lambda: ctx.gemm(A, B, C, n, n, n, dtype=_capi.GAT_BF16X3), 10, 2)
+        ms_x3 = timed_median(ctx, lambda: ctx.gemm(A, B, C, n, n, n, dtype=_capi.GAT_BF16X3), 10)
         tf_x3 = flops / (ms_x3 * 1e-3) / 1e12
         sec["bf16x3_8192"] = {"ms": round(ms_x3, 4), "TFLOP/s": round(tf_x3, 2),
-                              "vs_fp32_path": round(tf_x3 / tflops, 3),
+                              "vs_fp32_path": round(tf_x3 / tflops, 3), "timing": MEDIAN_NOTE,
                               "tile(wm,wn,tm,tn,bk,blk,wk)": list(ctx.tile_for(n, n, n, dtype=_capi.GAT_BF16X3)),
                               "note": "FP32 inputs split into bf16 hi/lo planes by a split pass (included in "
                                       "the time), 3 MFMA products per element pair; TFLOP/s counted as 2MNK"}
         a16, b16 = ctx.empty(n * n, np.uint16), ctx.empty(n * n, np.uint16)
         ctx.convert_bf16(A, a16)
         ctx.convert_bf16(B, b16)
-        ms_b, _ = timed(ctx, lambda: ctx.gemm(a16, b16, C, n, n, n, dtype=_capi.GAT_BF16), 10, 2)
+        ms_b = timed_median(ctx, lambda: ctx.gemm(a16, b16, C, n, n, n, dtype=_capi.GAT_BF16), 10)
         tf_b = flops / (ms_b * 1e-3) / 1e12
-        sec["bf16_8192"] = {"ms": round(ms_b, 4), "TFLOP/s": round(tf_b, 2), "frac_bf16_peak": round(tf_b / 2500, 4)}
+        sec["bf16_8192"] = {"ms": round(ms_b, 4), "TFLOP/s": round(tf_b, 2), "frac_bf16_peak": round(tf_b / 2500, 4),
+                            "timing": MEDIAN_NOTE}
         a16.free()
         b16.free()
         nb = (n // 128) * (n // 128)
@@ -623,10 +654,10 @@ def main():
         sA, sB = ctx.empty(nb, np.float32), ctx.empty(nb, np.float32)
         ms_q, _ = timed(ctx, lambda: (ctx.quantize_fp8(A, n, n, n, False, A8, sA),
                                       ctx.quantize_fp8(B, n, n, n, True, B8, sB)), 3, 1)
-        ms_f8, _ = timed(ctx, lambda: ctx.gemm_fp8(A8, sA, B8, sB, C, n, n, n), 10, 2)
+        ms_f8 = timed_median(ctx, lambda: ctx.gemm_fp8(A8, sA, B8, sB, C, n, n, n), 10)
         tf_f8 = flops / (ms_f8 * 1e-3) / 1e12
         sec["fp8_8192"] = {"ms": round(ms_f8, 4), "TFLOP/s": round(tf_f8, 2), "frac_fp8_peak": round(tf_f8 / 5000, 4),
-                           "quantize_ms_both_operands": round(ms_q, 4)}
+                           "quantize_ms_both_operands": round(ms_q, 4), "timing": MEDIAN_NOTE}
         for a in (A8, B8, sA, sB):
             a.free()
     for a in (A, B, C):

@@ -93,7 +93,7 @@ SIGNATURES = {
     "gat_grouped_shape": (_c_int, [_c_vp, _c_sz]),
     "gat_grouped_destroy": (None, [_c_vp]),
     "gat_gemm_grouped": (_c_int, [_c_vp, _c_int, ctypes.POINTER(GemmDesc), _c_sz]),
-    "gat_grouped_tune": (_c_int, [_c_vp, _c_int, ctypes.POINTER(GemmDesc), _c_sz, _c_int]),
+    "gat_grouped_tune": (_c_int, [_c_vp, _c_int, ctypes.POINTER(GemmDesc), _c_sz, _c_int, ctypes.c_char_p]),
     "gat_create": (_c_vp, [_c_int]),
     "gat_destroy": (None, [_c_vp]),
     "gat_last_error": (ctypes.c_char_p, []),

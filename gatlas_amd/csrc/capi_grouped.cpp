@@ -1,12 +1,16 @@
 // extern "C" layer of the grouped GEMM (persistent launch, HIP graph) and
 // its per-size-bucket tile tuning.
+#include <algorithm>
+#include <cmath>
 #include <map>
 #include <memory>
+#include <sstream>
 #include <vector>
 
 #include "../../include/gatlas.h"
 #include "capi_internal.hpp"
 #include "grouped_plan.hpp"
+#include "host/bench.hpp"
 
 using namespace gat;
 
@@ -68,7 +72,8 @@ int gat_gemm_grouped(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t
     return rc;
 }
 
-int gat_grouped_tune(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t count, int reps) {
+int gat_grouped_tune(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t count, int reps,
+                     const char* journal_path) {
     if (!ctx || (count && !descs)) {
         set_error("gat_grouped_tune: null context or descriptors");
         return GAT_ERR_INVALID;
@@ -80,6 +85,12 @@ int gat_grouped_tune(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t
     Context& c = ctx->impl;
     c.make_current();
     if (reps < 1) reps = 1;
+    std::unique_ptr<Journal> journal;
+    if (journal_path) {
+        journal.reset(new Journal(journal_path));
+        journal->loadMemo();
+    }
+    const char* form = dtype == GAT_F32 ? "groupedf32" : dtype == GAT_BF16 ? "groupedbf16" : "groupedbf16x3";
     std::map<BucketKey, size_t> rep;  // one representative problem per size bucket
     for (size_t i = 0; i < count; ++i) {
         GemmArgs a;
@@ -111,26 +122,48 @@ int gat_grouped_tune(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t
             reps_d[i] = GroupedDesc{d.transA ? 1 : 0, d.transB ? 1 : 0, (int64_t)d.M, (int64_t)d.N, (int64_t)d.K,
                                     1.0, d.A, (int64_t)d.lda, d.B, (int64_t)d.ldb, 0.0,
                                     (float*)scratch + i * d.M * d.N, (int64_t)d.N};
-        float best = 0.f;
-        int best_shape = 0;
-        for (int s = 0; s < 2; ++s) {
-            const std::vector<int> shapes(r, s);
-            GroupedPlan plan;
-            rc = plan.create(c, dtype, reps_d.data(), r, false, shapes.data());
-            if (rc) return rc;
-            rc = plan.run();  // warm-up
-            if (rc) return rc;
-            GAT_HIP_TRY(hipEventRecord(c.ev_start, c.stream));
-            for (int k = 0; k < reps; ++k) {
-                rc = plan.run();
+        double best = 0;
+        int best_shape = -1;
+        std::vector<std::pair<std::vector<std::pair<int, int>>, std::vector<std::pair<int, int>>>> seen;
+        for (int s = 0; s < GP_NSHAPES; ++s) {
+            // shapes that tile this problem alike (e.g. 128 and 64 rows when
+            // M <= 64) are timed once
+            std::vector<std::pair<int, int>> rs, cs;
+            GroupedPlan::tile_spans((int64_t)d.M, gp_shape_bm(s), rs);
+            GroupedPlan::tile_spans((int64_t)d.N, gp_shape_bn(s), cs);
+            bool dup = false;
+            for (const auto& t : seen) dup = dup || (t.first == rs && t.second == cs);
+            if (dup) continue;
+            seen.emplace_back(rs, cs);
+            std::ostringstream key;
+            key << form << "_" << (d.transA ? 1 : 0) << "_" << (d.transB ? 1 : 0) << "_" << d.M << "_" << d.N << "_"
+                << d.K << "_" << s << "_";
+            double us = journal ? journal->memoTime(key.str(), 0) : -1;  // a journaled trial is not re-run
+            if (us < 0) {
+                if (journal) journal->takeMemo(key.str(), Journal::RUN_IN_PROGRESS);
+                const std::vector<int> shapes(r, s);
+                GroupedPlan plan;
+                rc = plan.create(c, dtype, reps_d.data(), r, false, shapes.data());
                 if (rc) return rc;
+                rc = plan.run();  // warm-up
+                if (rc) return rc;
+                GAT_HIP_TRY(hipEventRecord(c.ev_start, c.stream));
+                for (int k = 0; k < reps; ++k) {
+                    rc = plan.run();
+                    if (rc) return rc;
+                }
+                GAT_HIP_TRY(hipEventRecord(c.ev_stop, c.stream));
+                GAT_HIP_TRY(hipEventSynchronize(c.ev_stop));
+                float ms = 0.f;
+                GAT_HIP_TRY(hipEventElapsedTime(&ms, c.ev_start, c.ev_stop));
+                us = std::max(1.0, std::round(1000.0 * ms / reps));
+                if (journal) {
+                    journal->takeMemo(key.str(), (int)us);
+                    journal->takeMemo(key.str(), Journal::RUN_OK);
+                }
             }
-            GAT_HIP_TRY(hipEventRecord(c.ev_stop, c.stream));
-            GAT_HIP_TRY(hipEventSynchronize(c.ev_stop));
-            float ms = 0.f;
-            GAT_HIP_TRY(hipEventElapsedTime(&ms, c.ev_start, c.ev_stop));
-            if (s == 0 || ms < best) {
-                best = ms;
+            if (best_shape < 0 || us < best) {
+                best = us;
                 best_shape = s;
             }
         }

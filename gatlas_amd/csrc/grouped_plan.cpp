@@ -15,11 +15,23 @@ GroupedPlan::~GroupedPlan() {
     if (d_counter_) hipFree(d_counter_);
 }
 
+// A single-problem configuration with the numerics of a grouped tile shape
+// (32x32 blocks, no k-split): every shape accumulates each element's
+// products in the same k order, so the 128 x 64 shapes map to the 128 x 128
+// configuration, and a grouped problem's bits equal a single launch's.
 MfmaCfg GroupedPlan::shape_cfg(int dtype, int shape) {
-    if (dtype == DT_BF16X3) return shape == 0 ? MfmaCfg{2, 2, 2, 2, 16, 32} : MfmaCfg{2, 2, 1, 1, 32, 32};
-    if (dtype == DT_BF16)
-        return shape == 0 ? MfmaCfg{2, 2, 2, 2, 32, 32} : MfmaCfg{2, 2, 1, 1, 64, 32};
-    return shape == 0 ? MfmaCfg{2, 2, 2, 2, 32, 32} : MfmaCfg{2, 2, 1, 1, 32, 32};
+    const bool small = shape == 1;
+    if (dtype == DT_BF16X3) return small ? MfmaCfg{2, 2, 1, 1, 32, 32} : MfmaCfg{2, 2, 2, 2, 16, 32};
+    if (dtype == DT_BF16) return small ? MfmaCfg{2, 2, 1, 1, 64, 32} : MfmaCfg{2, 2, 2, 2, 32, 32};
+    return small ? MfmaCfg{2, 2, 1, 1, 32, 32} : MfmaCfg{2, 2, 2, 2, 32, 32};
+}
+
+void GroupedPlan::tile_spans(int64_t extent, int T, std::vector<std::pair<int, int>>& spans) {
+    spans.clear();
+    for (int64_t o = 0; o < extent; o += T) {
+        const int64_t rest = extent - o;
+        spans.emplace_back((int)o, (T == 128 && rest <= 64) ? 64 : T);
+    }
 }
 
 int GroupedPlan::form_of(int dtype) {
@@ -40,8 +52,8 @@ int GroupedPlan::choose_shape(const Context& ctx, int dtype, const GroupedDesc& 
     a.K = d.K;
     int s = 0;
     if (ctx.get_grouped_shape(bucket_of(dtype, a), &s)) return s;
-    // untuned: 128 x 128 tiles once both sides fill them, else 64 x 64
-    return (d.M >= 256 && d.N >= 256) ? 0 : 1;
+    // untuned: 128-wide sides where a side fills two of them, else 64
+    return gp_shape_of(d.M >= 256 ? 128 : 64, d.N >= 256 ? 128 : 64);
 }
 
 int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* d, size_t n,
@@ -58,10 +70,14 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
     // A unit is a band of tile rows of one problem: a problem of few tiles is
     // one unit, a larger one is cut into up to GP_XCDS bands.
     struct Unit {
-        double cost, tcost;
-        int p, T, r0, r1, tn;
+        double cost, kcost;
+        int p, r0, r1;
     };
     std::vector<Unit> units;
+    // per problem: (offset, extent) of its tile rows and columns
+    std::vector<std::vector<std::pair<int, int>>> rows, cols;
+    rows.reserve(n);
+    cols.reserve(n);
     for (size_t i = 0; i < n; ++i) {
         const GroupedDesc& g = d[i];
         if (g.M < 0 || g.N < 0 || g.K < 0 || g.M > INT32_MAX || g.N > INT32_MAX || g.K > INT32_MAX ||
@@ -82,19 +98,30 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
         p.M = (int)g.M;
         p.N = (int)g.N;
         p.K = (int)g.K;
-        p.flags = (g.transA ? GP_TA : 0) | (g.transB ? GP_TB : 0) | (vec ? GP_VEC : 0) | (s << GP_SHAPE_SHIFT);
+        p.flags = (g.transA ? GP_TA : 0) | (g.transB ? GP_TB : 0) | (vec ? GP_VEC : 0);
         p.lda = g.lda;
         p.ldb = g.ldb;
         p.ldc = g.ldc;
         p.alpha = (float)g.alpha;
         p.beta = (float)g.beta;
-        const int T = s == 0 ? 128 : 64;
-        const int tm = (int)((g.M + T - 1) / T), tn = (int)((g.N + T - 1) / T);
-        const double tc = (double)T * T * (double)(g.K + 32);  // cost model of one tile
+        if (s < 0 || s >= GP_NSHAPES) {
+            set_error("grouped gemm: tile shape out of range");
+            return ST_INVALID;
+        }
+        rows.emplace_back();
+        cols.emplace_back();
+        tile_spans(g.M, gp_shape_bm(s), rows.back());
+        tile_spans(g.N, gp_shape_bn(s), cols.back());
+        const int tm = (int)rows.back().size(), tn = (int)cols.back().size();
+        const double kc = (double)(g.K + 32);  // cost model: tile area times (K + 32)
         const int nb = (tm * tn <= 32) ? 1 : std::min(GP_XCDS, tm);
         for (int b = 0; b < nb && tm > 0 && tn > 0; ++b) {
             const int r0 = b * tm / nb, r1 = (b + 1) * tm / nb;
-            if (r1 > r0) units.push_back(Unit{(double)(r1 - r0) * tn * tc, tc, (int)i, T, r0, r1, tn});
+            if (r1 <= r0) continue;
+            double area = 0;
+            for (int r = r0; r < r1; ++r)
+                for (const auto& c : cols.back()) area += (double)rows.back()[r].second * c.second;
+            units.push_back(Unit{area * kc, kc, (int)i, r0, r1});
         }
         flops += 2.0 * (double)g.M * (double)g.N * (double)g.K;
     }
@@ -103,7 +130,7 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
     std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& b) { return a.cost > b.cost; });
     struct Tile {
         double cost;
-        int p, m0, n0;
+        int p, m0, n0, shape;
     };
     std::vector<Tile> lists[GP_XCDS];
     double load[GP_XCDS] = {0};
@@ -113,8 +140,12 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
             if (load[y] < load[x]) x = y;
         load[x] += u.cost;
         // column-major inside the band: consecutive tiles share a B panel
-        for (int c = 0; c < u.tn; ++c)
-            for (int r = u.r0; r < u.r1; ++r) lists[x].push_back(Tile{u.tcost, u.p, r * u.T, c * u.T});
+        for (const auto& c : cols[u.p])
+            for (int r = u.r0; r < u.r1; ++r) {
+                const auto& rw = rows[u.p][r];
+                lists[x].push_back(Tile{(double)rw.second * c.second * u.kcost, u.p, rw.first, c.first,
+                                        gp_shape_of(rw.second, c.second)});
+            }
     }
     work.clear();
     for (int x = 0; x < GP_XCDS; ++x) {
@@ -122,7 +153,7 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
         std::stable_sort(lists[x].begin(), lists[x].end(),
                          [](const Tile& a, const Tile& b) { return a.cost > b.cost; });
         q.off[x] = (int)work.size();
-        for (const Tile& t : lists[x]) work.push_back(make_int4(t.p, t.m0, t.n0, 0));
+        for (const Tile& t : lists[x]) work.push_back(make_int4(t.p, t.m0, t.n0, t.shape));
     }
     q.off[GP_XCDS] = (int)work.size();
     return ST_OK;

@@ -3,6 +3,7 @@
 // the (counter reset, persistent kernel) pair.
 #pragma once
 
+#include <utility>
 #include <vector>
 
 #include "context.hpp"
@@ -35,17 +36,22 @@ public:
     size_t tiles() const { return nwork_; }
     double flops() const { return flops_; }
     int grid() const { return grid_; }
-    // tile shape chosen for problem i (0 = 128x128, 1 = 64x64)
+    // interior tile shape of problem i (0 = 128x128, 1 = 64x64, 2 = 128x64,
+    // 3 = 64x128); its edge strips may take a narrower shape
     int shape(size_t i) const { return shapes_[i]; }
 
     // shape for one problem: the context's tuned choice for its size bucket
     // when one exists, else a size rule
     static int choose_shape(const Context& ctx, int dtype, const GroupedDesc& d);
     static MfmaCfg shape_cfg(int dtype, int shape);
+    // (offset, extent) of the tiles of width T along a side of `extent`;
+    // with T = 128 a last tile over 64 or fewer elements is 64 wide
+    static void tile_spans(int64_t extent, int T, std::vector<std::pair<int, int>>& spans);
     // grouped-kernel precision form of a dtype (-1: not supported)
     static int form_of(int dtype);
-    // Device descriptors and the cost-ordered (problem, m0, n0) tile list
-    // for n problems; also the tile shape per problem and the total flops.
+    // Device descriptors and the cost-ordered (problem, m0, n0, shape) tile
+    // list for n problems; also the interior shape per problem and the total
+    // flops.
     static int build_tables(const Context& ctx, int dtype, const GroupedDesc* d, size_t n,
                             std::vector<GroupedProblemDev>& probs, std::vector<int4>& work, GroupedQueues& q,
                             std::vector<int>& shapes, double& flops, const int* force_shapes = nullptr);

@@ -68,15 +68,23 @@ int Journal::memoRunState(const KernelInterface& k, const std::vector<size_t>& p
 }
 
 int Journal::memoTime(const KernelInterface& k, const std::vector<size_t>& params, size_t trial) const {
-    auto it = times_.find(key(k, params));
+    return memoTime(key(k, params), trial);
+}
+
+int Journal::memoTime(const std::string& key, size_t trial) const {
+    auto it = times_.find(key);
     if (it == times_.end() || trial >= it->second.size()) return -1;
     return (int)it->second[trial];
 }
 
 bool Journal::takeMemo(const KernelInterface& k, const std::vector<size_t>& params, int value) const {
+    return takeMemo(key(k, params), value);
+}
+
+bool Journal::takeMemo(const std::string& key, int value) const {
     std::ofstream out(file_.c_str(), std::ios::app);
     if (!out.is_open()) return false;
-    out << key(k, params) << "\t" << value << std::endl;  // flushed: the record must survive a crash
+    out << key << "\t" << value << std::endl;  // flushed: the record must survive a crash
     return true;
 }
 

@@ -37,6 +37,10 @@ public:
     int memoRunState(const KernelInterface& k, const std::vector<size_t>& params) const;
     int memoTime(const KernelInterface& k, const std::vector<size_t>& params, size_t trial) const;
     bool takeMemo(const KernelInterface& k, const std::vector<size_t>& params, int value) const;
+    // the same records under a key built by the caller (trials that have no
+    // KernelInterface, e.g. the grouped kernel's tile shapes)
+    int memoTime(const std::string& key, size_t trial) const;
+    bool takeMemo(const std::string& key, int value) const;
     const std::string& file() const { return file_; }
 
 private:

@@ -16,8 +16,9 @@
 // and replays of a captured graph holding this one kernel node -- start
 // from zeroed counters.  They must be zero before the first launch.
 //
-// Tile shapes: 0 = 128 x 128 (2 x 2 waves of 2 x 2 32x32 blocks),
-//              1 = 64 x 64 (2 x 2 waves of one 32x32 block).
+// Tile shapes (grouped.hpp): 0 = 128 x 128 (2 x 2 waves of 2 x 2 32x32
+// blocks), 1 = 64 x 64 (one block per wave), 2 = 128 x 64 and 3 = 64 x 128
+// (2 x 1 and 1 x 2 blocks per wave); each work entry carries its tile's.
 // BF16 interior tiles of problems with K >= 64 a multiple of 8, and FP32
 // interior tiles of problems with K >= 32 a multiple of 16, with 16-byte aligned
 // rows, stage straight into LDS (bf16_dma_tile.hpp, sgemm_dma_tile.hpp);
@@ -25,7 +26,8 @@
 // split-BF16x3 form splits FP32 operands into bf16 hi/lo planes while a
 // stage is stored to LDS (bf16_tile.hpp, MODE_X3): the routine of the
 // single-problem x3 kernel of that tile shape, staged 16 (128 x 128) or 32
-// (64 x 64) deep so that three groups fit a CU.  The staging depth does not
+// (64 x 64) deep so that three groups fit a CU (the 128 x 64 shapes 16
+// deep).  The staging depth does not
 // change an x3 tile's bits (its products run in 16-deep steps in k order
 // at any depth), so a problem's result equals a single launch's.
 #include "bf16_dma_tile.hpp"
@@ -39,84 +41,93 @@ namespace gat {
 // Interior FP32 tiles of problems with K >= 32 a multiple of 16 and
 // 16-byte aligned rows stage straight into LDS (sgemm_dma_tile.hpp, the
 // same bits).
-template <bool TA, bool TB>
-__device__ __forceinline__ void f32_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0, float* smem) {
+template <int TM, int TN, bool TA, bool TB>
+__device__ __forceinline__ void f32_tile_t(const GroupedProblemDev& p, int m0, int n0, float* smem) {
+    constexpr int BM = 2 * TM * 32, BN = 2 * TN * 32;
     const bool vec = (p.flags & GP_VEC) != 0;
     const bool dma = vec && p.K >= 32 && p.K % 16 == 0;
     const float* A = (const float*)p.A;
     const float* B = (const float*)p.B;
-    if (shape == 0) {
-        if (dma && m0 + 128 <= p.M && n0 + 128 <= p.N)
-            sgemm_tile_dma<2, 2, 2, 2, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0,
-                                               n0, smem);
-        else
-            sgemm_tile<2, 2, 2, 2, 32, 32, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
-                                                   m0, n0, vec, smem);
-    } else {
-        if (dma && m0 + 64 <= p.M && n0 + 64 <= p.N)
-            sgemm_tile_dma<2, 2, 1, 1, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0,
-                                               n0, smem);
-        else
-            sgemm_tile<2, 2, 1, 1, 32, 32, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
-                                                   m0, n0, vec, smem);
-    }
+    if (dma && m0 + BM <= p.M && n0 + BN <= p.N)
+        sgemm_tile_dma<2, 2, TM, TN, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0, n0,
+                                             smem);
+    else
+        sgemm_tile<2, 2, TM, TN, 32, 32, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0,
+                                                 n0, vec, smem);
+}
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void f32_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0, float* smem) {
+    if (shape == 0) f32_tile_t<2, 2, TA, TB>(p, m0, n0, smem);
+    else if (shape == 1) f32_tile_t<1, 1, TA, TB>(p, m0, n0, smem);
+    else if (shape == 2) f32_tile_t<2, 1, TA, TB>(p, m0, n0, smem);
+    else f32_tile_t<1, 2, TA, TB>(p, m0, n0, smem);
+}
+
+// BF16 interior tiles of problems with K >= 64 a multiple of 8 and 16-byte
+// aligned rows stage straight into LDS; the others through registers, BK
+// deep.
+template <int TM, int TN, int BK, bool TA, bool TB>
+__device__ __forceinline__ void bf16_tile_t(const GroupedProblemDev& p, int m0, int n0, __bf16* smem) {
+    constexpr int BM = 2 * TM * 32, BN = 2 * TN * 32;
+    const bool vec = (p.flags & GP_VEC) != 0;
+    const bool dma = vec && p.K >= 64 && p.K % 8 == 0;
+    if (dma && m0 + BM <= p.M && n0 + BN <= p.N)
+        bf16_tile_dma<2, 2, TM, TN, TA, TB>((const __bf16*)p.A, (const __bf16*)p.B, p.C, p.M, p.N, p.K, p.lda,
+                                            p.ldb, p.ldc, p.alpha, p.beta, m0, n0, smem);
+    else
+        bf16_tile<2, 2, TM, TN, BK, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+                                                       p.beta, m0, n0, vec, smem);
 }
 
 template <bool TA, bool TB>
 __device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0,
                                                 __bf16* smem) {
-    const bool vec = (p.flags & GP_VEC) != 0;
-    // interior tiles with K >= 64 a multiple of 8 and 16-byte aligned rows
-    // stage straight into LDS
-    const bool dma = vec && p.K >= 64 && p.K % 8 == 0;
-    if (shape == 0) {
-        if (dma && m0 + 128 <= p.M && n0 + 128 <= p.N)
-            bf16_tile_dma<2, 2, 2, 2, TA, TB>((const __bf16*)p.A, (const __bf16*)p.B, p.C, p.M, p.N, p.K, p.lda,
-                                              p.ldb, p.ldc, p.alpha, p.beta, m0, n0, smem);
-        else
-            bf16_tile<2, 2, 2, 2, 32, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc,
-                                                         p.alpha, p.beta, m0, n0, vec, smem);
-    } else {
-        if (dma && m0 + 64 <= p.M && n0 + 64 <= p.N)
-            bf16_tile_dma<2, 2, 1, 1, TA, TB>((const __bf16*)p.A, (const __bf16*)p.B, p.C, p.M, p.N, p.K, p.lda,
-                                              p.ldb, p.ldc, p.alpha, p.beta, m0, n0, smem);
-        else
-            bf16_tile<2, 2, 1, 1, 64, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc,
-                                                         p.alpha, p.beta, m0, n0, vec, smem);
-    }
+    if (shape == 0) bf16_tile_t<2, 2, 32, TA, TB>(p, m0, n0, smem);
+    else if (shape == 1) bf16_tile_t<1, 1, 64, TA, TB>(p, m0, n0, smem);
+    else if (shape == 2) bf16_tile_t<2, 1, 32, TA, TB>(p, m0, n0, smem);
+    else bf16_tile_t<1, 2, 32, TA, TB>(p, m0, n0, smem);
+}
+
+template <int TM, int TN, int BK, bool TA, bool TB>
+__device__ __forceinline__ void x3_tile_t(const GroupedProblemDev& p, int m0, int n0, __bf16* smem) {
+    bf16_tile<2, 2, TM, TN, BK, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
+                                                 m0, n0, (p.flags & GP_VEC) != 0, smem);
 }
 
 template <bool TA, bool TB>
 __device__ __forceinline__ void x3_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0,
                                               __bf16* smem) {
-    const bool vec = (p.flags & GP_VEC) != 0;
-    if (shape == 0)
-        bf16_tile<2, 2, 2, 2, 16, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
-                                                   p.beta, m0, n0, vec, smem);
-    else
-        bf16_tile<2, 2, 1, 1, 32, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
-                                                   p.beta, m0, n0, vec, smem);
+    if (shape == 0) x3_tile_t<2, 2, 16, TA, TB>(p, m0, n0, smem);
+    else if (shape == 1) x3_tile_t<1, 1, 32, TA, TB>(p, m0, n0, smem);
+    else if (shape == 2) x3_tile_t<2, 1, 16, TA, TB>(p, m0, n0, smem);
+    else x3_tile_t<1, 2, 16, TA, TB>(p, m0, n0, smem);
 }
 
-// LDS of the tile buffers (the larger of the two shapes).  The kernel
+// LDS of the tile buffers (the largest shape: 128 x 128; the 128 x 64
+// shapes take less at the same depth).  The kernel
 // declares no static LDS, so the dynamic block -- and every 16-byte tile
 // access in it -- starts aligned.
+constexpr size_t gp_max(size_t a, size_t b) { return a > b ? a : b; }
+
 template <int FORM>
 constexpr size_t grouped_tile_bytes() {
-    typedef Bf16Shape<2, 2, 2, 2, 16, MODE_X3> X0;
-    typedef Bf16Shape<2, 2, 1, 1, 32, MODE_X3> X1;
-    constexpr size_t a = FORM == GP_FORM_F32    ? SgemmShape<2, 2, 2, 2, 32, 32>::SMEM
-                         : FORM == GP_FORM_BF16 ? Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM
-                                                : X0::SMEM;
-    constexpr size_t b = FORM == GP_FORM_F32    ? SgemmShape<2, 2, 1, 1, 32, 32>::SMEM
-                         : FORM == GP_FORM_BF16 ? Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM
-                                                : X1::SMEM;
-    // direct-to-LDS stages of shape 0
-    constexpr size_t c = FORM == GP_FORM_BF16 ? Bf16DmaShape<2, 2, 2, 2>::SMEM
-                         : FORM == GP_FORM_F32 ? SgemmDmaShape<2, 2, 2, 2>::SMEM
-                                               : 0;
-    constexpr size_t ab = a > b ? a : b;
-    return ((ab > c ? ab : c) + 15) / 16 * 16;
+    // register-staged stages of shapes 0, 1 and 2 (3 takes what 2 does)
+    constexpr size_t reg = FORM == GP_FORM_F32
+                               ? gp_max(gp_max(SgemmShape<2, 2, 2, 2, 32, 32>::SMEM, SgemmShape<2, 2, 1, 1, 32, 32>::SMEM),
+                                        SgemmShape<2, 2, 2, 1, 32, 32>::SMEM)
+                           : FORM == GP_FORM_BF16
+                               ? gp_max(gp_max(Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM,
+                                               Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM),
+                                        Bf16Shape<2, 2, 2, 1, 32, MODE_BF16>::SMEM)
+                               : gp_max(gp_max(Bf16Shape<2, 2, 2, 2, 16, MODE_X3>::SMEM,
+                                               Bf16Shape<2, 2, 1, 1, 32, MODE_X3>::SMEM),
+                                        Bf16Shape<2, 2, 2, 1, 16, MODE_X3>::SMEM);
+    // direct-to-LDS stages (the 128 x 128 shape's are the largest)
+    constexpr size_t dma = FORM == GP_FORM_BF16 ? Bf16DmaShape<2, 2, 2, 2>::SMEM
+                           : FORM == GP_FORM_F32 ? SgemmDmaShape<2, 2, 2, 2>::SMEM
+                                                 : 0;
+    return (gp_max(reg, dma) + 15) / 16 * 16;
 }
 
 // Work-groups per CU, as LDS allows (160 KiB per CU, tile buffers plus the
@@ -161,7 +172,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grouped_wgs
         if (w < 0) break;  // every queue is drained
         const int4 t = work[w];
         const GroupedProblemDev p = probs[t.x];
-        const int shape = (p.flags >> GP_SHAPE_SHIFT) & 15;
+        const int shape = t.w;
         const int tr = p.flags & (GP_TA | GP_TB);
         if constexpr (FORM == GP_FORM_F32) {
             if (tr == 0) f32_tile_shape<false, false>(p, shape, t.y, t.z, gsmem);

@@ -290,8 +290,8 @@ int Server::launch_batch(int kind, std::vector<int>& pend, int slot, Launch& L) 
         size_t tiles = 0, take = 0;
         for (; take < pend.size() && (int)take < cfg_.max_batch; ++take) {
             const ServeReq& r = reqs_[pend[take]];
-            const int64_t T = r.shape == 0 ? 128 : 64;
-            const size_t nt = (size_t)((r.M + T - 1) / T) * (size_t)((r.N + T - 1) / T);
+            const int64_t TM = gp_shape_bm(r.shape), TN = gp_shape_bn(r.shape);
+            const size_t nt = (size_t)((r.M + TM - 1) / TM) * (size_t)((r.N + TN - 1) / TN);
             if (take && tiles + nt > work_cap_) break;
             tiles += nt;
             descs.push_back(GroupedDesc{r.ta, r.tb, r.M, r.N, r.K, 1.0, pool_ptr(r.aoff), r.lda, pool_ptr(r.boff),

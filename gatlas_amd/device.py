@@ -220,9 +220,13 @@ class Context:
     def tuned_clear(self):
         check(self.lib.gat_tuned_clear(self.h), "gat_tuned_clear")
 
-    def grouped_tune(self, problems, dtype=_capi.GAT_F32, reps=2):
+    def grouped_tune(self, problems, dtype=_capi.GAT_F32, reps=2, journal=None):
+        """Tunes the grouped tile shape of every size bucket the problems
+        touch; `journal` (a path) records every trial and replays those
+        already recorded.  Returns the number of buckets tuned."""
         descs = self.gemm_descs(problems)
-        n = self.lib.gat_grouped_tune(self.h, dtype, descs, len(problems), reps)
+        n = self.lib.gat_grouped_tune(self.h, dtype, descs, len(problems), reps,
+                                      None if journal is None else str(journal).encode())
         if n < 0:
             raise _capi.GatError("grouped_tune: " + _capi.last_error())
         return n

@@ -219,17 +219,25 @@ gat_grouped *gat_grouped_create(gat_ctx *ctx, int dtype, const gat_gemm_desc *de
 /* One persistent launch (a graph replay when created with use_graph). */
 int gat_grouped_run(gat_grouped *g);
 int gat_grouped_info(gat_grouped *g, size_t *tiles, double *flops, int *grid);
-/* Tile shape chosen for problem i: 0 = 128x128, 1 = 64x64. */
+/* Interior tile shape of problem i: 0 = 128x128, 1 = 64x64, 2 = 128x64,
+ * 3 = 64x128 (M x N); a last tile row or column over 64 or fewer rows or
+ * columns of a 128-wide shape is computed 64 wide. */
 int gat_grouped_shape(gat_grouped *g, size_t i);
 void gat_grouped_destroy(gat_grouped *g);
 /* create + run + synchronise + destroy. */
 int gat_gemm_grouped(gat_ctx *ctx, int dtype, const gat_gemm_desc *descs, size_t count);
-/* Times both tile shapes for one problem per size bucket, inside a full
- * persistent launch of replicas of that problem (reading its A and B, writing
- * library scratch: the caller's C is not touched), and installs the faster in
- * the context's grouped-shape table (single-problem GEMMs keep their own
- * tuned tiles); returns the number of buckets tuned. */
-int gat_grouped_tune(gat_ctx *ctx, int dtype, const gat_gemm_desc *descs, size_t count, int reps);
+/* Times the four tile shapes (gat_grouped_shape) for one problem per size
+ * bucket, inside a full persistent launch of replicas of that problem
+ * (reading its A and B, writing library scratch: the caller's C is not
+ * touched), and installs the fastest in the context's grouped-shape table
+ * (single-problem GEMMs keep their own tuned tiles); shapes that tile the
+ * problem alike are timed once.  journal_path (may be NULL): every trial is
+ * journaled as the reference's benchmark journal does (GatlasBenchmark.cpp:
+ * 44-137 format, key "grouped<form>_<tA>_<tB>_<M>_<N>_<K>_<shape>_": -3, the
+ * microseconds of one launch, -4), and a trial already journaled is not run
+ * again.  Returns the number of buckets tuned. */
+int gat_grouped_tune(gat_ctx *ctx, int dtype, const gat_gemm_desc *descs, size_t count, int reps,
+                     const char *journal_path);
 
 /* Grouped GEMV (FP32, A row-major M x N): y = alpha*A*x + beta*y for many
  * problems in one launch; each result is bit-identical to gat_gemv's. */

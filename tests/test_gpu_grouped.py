@@ -23,9 +23,14 @@ from tests.helpers import bf16_bits_to_f32, uniform_host
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
-SHAPE_CFG = {_capi.GAT_F32: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 32, 32)],
-             _capi.GAT_BF16: [(2, 2, 2, 2, 32, 32), (2, 2, 1, 1, 64, 32)],
-             _capi.GAT_BF16X3: [(2, 2, 2, 2, 16, 32), (2, 2, 1, 1, 32, 32)]}
+# single-problem tile of each grouped shape's numerics (32x32 blocks, no
+# k-split: every shape sums an element's products in the same k order, so the
+# 128 x 64 shapes compare against the 128 x 128 tile)
+_BIG = {_capi.GAT_F32: (2, 2, 2, 2, 32, 32), _capi.GAT_BF16: (2, 2, 2, 2, 32, 32),
+        _capi.GAT_BF16X3: (2, 2, 2, 2, 16, 32)}
+_SMALL = {_capi.GAT_F32: (2, 2, 1, 1, 32, 32), _capi.GAT_BF16: (2, 2, 1, 1, 64, 32),
+          _capi.GAT_BF16X3: (2, 2, 1, 1, 32, 32)}
+SHAPE_CFG = {dt: [_BIG[dt], _SMALL[dt], _BIG[dt], _BIG[dt]] for dt in _BIG}
 FORMS = [_capi.GAT_F32, _capi.GAT_BF16, _capi.GAT_BF16X3]
 FORM_IDS = ["f32", "bf16", "bf16x3"]
 
@@ -219,3 +224,42 @@ def test_grouped_512_workload(ctx, dtype):
         cw, _, _ = ora.error_metrics(np.ascontiguousarray(got[i].reshape(M, N)[rows]), r64, mag)
         assert cw <= TOL, (i, dims[i], cw)
     print("[grouped512] dtype %d %s shapes(128x128)=%d" % (dtype, g.info(), shapes.count(0)))
+
+
+def test_grouped_tune_journal_replay(ctx, tmp_path):
+    """The grouped tuner journals every shape trial (-3, microseconds, -4 per
+    key, the reference's benchmark journal records) and, given the same
+    journal again, runs no trial and installs the same shapes."""
+    rng = np.random.default_rng(31)
+    dims = [(int(M), int(N), int(K), int(ta), int(tb))
+            for M, N, K, ta, tb in zip(rng.choice([48, 200, 640, 1100], 8), rng.choice([64, 130, 700], 8),
+                                       rng.choice([96, 512], 8), rng.integers(0, 2, 8), rng.integers(0, 2, 8))]
+    _say("grouped tune journal", dims)
+    host, probs = _make_group(ctx, dims, 17, general=False)
+    jnl = tmp_path / "grouped.jnl"
+    try:
+        n1 = ctx.grouped_tune(probs, _capi.GAT_F32, journal=jnl)
+        g = ctx.grouped(probs, _capi.GAT_F32, use_graph=False)
+        s1 = g.shapes()
+        g.close()
+        recs = [ln.split("\t") for ln in jnl.read_text().splitlines()]
+        ctx.tuned_clear()
+        n2 = ctx.grouped_tune(probs, _capi.GAT_F32, journal=jnl)
+        g = ctx.grouped(probs, _capi.GAT_F32, use_graph=False)
+        s2 = g.shapes()
+        g.close()
+    finally:
+        ctx.tuned_clear()
+    assert n1 == n2 >= 1
+    keys = {}
+    for k, v in recs:
+        keys.setdefault(k, []).append(int(v))
+    for k, vals in keys.items():
+        assert k.startswith("groupedf32_") and k.endswith("_"), k
+        assert int(k.split("_")[-2]) in (0, 1, 2, 3), k
+        assert vals[0] == -3 and vals[-1] == -4 and len(vals) == 3 and vals[1] > 0, (k, vals)
+    # at most four shapes per bucket, and every bucket timed at least one
+    assert n1 <= len(keys) <= 4 * n1
+    assert len(jnl.read_text().splitlines()) == len(recs)  # replay appended nothing
+    assert s1 == s2
+    assert all(0 <= s <= 3 for s in s1)
