@@ -121,6 +121,22 @@ struct StageIO<MODE_BF16, ROWS, BK, NT, TRANS> {
             }
         }
     }
+    // Interior tiles (every stage whole): each vector's element offset from
+    // the stage's first element, computed once per tile; a stage is then one
+    // 16-byte load per vector.
+    int64_t goff[NV];
+    __device__ __forceinline__ void prep(int64_t ld, int r0, int tid) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            int r, k;
+            coords(q, tid, r, k);
+            goff[q] = !TRANS ? (int64_t)(r0 + r) * ld + k : (int64_t)k * ld + r0 + r;
+        }
+    }
+    __device__ __forceinline__ void load_at(const __bf16* __restrict__ base) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) reg[q] = gload<u32x4>(base + goff[q]);
+    }
     __device__ __forceinline__ void store(__bf16* hi, __bf16* /*lo*/, int tid) const {
         if constexpr (!TRANS) {
 #pragma unroll
@@ -190,6 +206,20 @@ struct StageIO<MODE_X3, ROWS, BK, NT, TRANS> {
                 }
             }
         }
+    }
+    // interior tiles: as the BF16 form's
+    int64_t goff[NV];
+    __device__ __forceinline__ void prep(int64_t ld, int r0, int tid) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            int r, k;
+            coords(q, tid, r, k);
+            goff[q] = !TRANS ? (int64_t)(r0 + r) * ld + k : (int64_t)k * ld + r0 + r;
+        }
+    }
+    __device__ __forceinline__ void load_at(const float* __restrict__ base) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) reg[q] = gload<f4v>(base + goff[q]);
     }
     __device__ __forceinline__ void store(__bf16* hi, __bf16* lo, int tid) const {
         if constexpr (!TRANS) {
@@ -266,8 +296,21 @@ __device__ __forceinline__ void bf16_tile(const void* __restrict__ Av, const voi
     IoA ioa;
     IoB iob;
     const int KT = (K + BK - 1) / BK;
+    // interior tile with whole stages: per-vector offsets computed once
+    const bool interior = full_mn && K % BK == 0;
+    if (interior) {
+        ioa.prep(lda, m0, tid);
+        iob.prep(ldb, n0, tid);
+    }
     auto fetch = [&](int t) {
         const int k0 = t * BK;
+        if (interior) {
+            // op(A) rows are k-contiguous unless A is transposed; op(B)^T rows
+            // are k-contiguous when B is transposed
+            ioa.load_at(A + (TA ? (int64_t)k0 * lda : (int64_t)k0));
+            iob.load_at(B + (TB ? (int64_t)k0 : (int64_t)k0 * ldb));
+            return;
+        }
         const bool full = full_mn && (k0 + BK <= K);
         ioa.load(A, lda, m0, k0, M, K, full, vec_ok, tid);
         iob.load(B, ldb, n0, k0, N, K, full, vec_ok, tid);
