@@ -344,7 +344,7 @@ int gat_tune_gemm(gat_ctx* ctx, int dtype, int transA, int transB, size_t M, siz
     if (best.empty()) return GAT_ERR_UNSUPPORTED;
     m->setParams(best);
     const std::vector<size_t> d = m->extraParamDetail();
-    const MfmaCfg c{(int)best[7], (int)best[8], (int)best[9], (int)best[10], kMfmaBK[d[0]], kMfmaBlk[d[1]],
+    const MfmaCfg c{(int)best[7], (int)best[8], (int)best[9], (int)best[10], mfma_depth(dtype, d[0]), kMfmaBlk[d[1]],
                     kMfmaWK[d[2]]};
     GemmArgs a;
     a.transA = transA;

@@ -11,6 +11,10 @@
 namespace gat {
 
 const int kMfmaBK[4] = {8, 16, 32, 64};
+// FP8 tiles are always 128 deep (one scale block), so their depth digit has
+// the single value 128; the other forms take kMfmaBK.
+size_t mfma_depths(int dtype) { return dtype == DT_FP8 ? 1 : 4; }
+int mfma_depth(int dtype, size_t i) { return dtype == DT_FP8 ? 128 : kMfmaBK[i]; }
 const int kMfmaBlk[2] = {32, 16};
 const int kMfmaWK[3] = {1, 2, 4};
 
@@ -221,7 +225,7 @@ public:
         family = FAM_MATMUL;
         variant = VAR_MFMA;
         dtype = dt;
-        xp_.addDigit(4);  // LDS depth bk in kMfmaBK
+        xp_.addDigit(mfma_depths(dt));  // LDS depth bk (mfma_depth)
         xp_.addDigit(2);  // block shape in kMfmaBlk
         xp_.addDigit(3);  // wave groups sharing each K tile in kMfmaWK
     }
@@ -242,7 +246,7 @@ public:
     MfmaCfg cfg() const {
         const auto d = extraParamDetail();
         return MfmaCfg{(int)groupHeight(), (int)groupWidth(), (int)blockHeight(), (int)blockWidth(),
-                       kMfmaBK[d.empty() ? 0 : d[0]], kMfmaBlk[d.size() < 2 ? 0 : d[1]],
+                       mfma_depth(dtype, d.empty() ? 0 : d[0]), kMfmaBlk[d.size() < 2 ? 0 : d[1]],
                        kMfmaWK[d.size() < 3 ? 0 : d[2]]};
     }
     bool validParams() const override {

@@ -48,6 +48,10 @@ int kernel_dtype(KernelInterface* k);
 
 // MFMA digit tables (LDS depth, block shape).
 extern const int kMfmaBK[4];
+// values of the MFMA kernels' depth digit for a precision form, and the
+// depth of digit value i
+size_t mfma_depths(int dtype);
+int mfma_depth(int dtype, size_t i);
 extern const int kMfmaBlk[2];
 extern const int kMfmaWK[3];
 

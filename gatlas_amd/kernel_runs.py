@@ -20,6 +20,17 @@ Workloads (inputs U[-1,1] from the device generator):
 --tile W_M,W_N,T_M,T_N,BK,BLK,WK installs that tile for the GEMM workloads'
 size bucket first (the tuned-table entry the autotuner would write), so one
 tile of the table can be profiled at the workload's size.
+
+Vendor reference points (PyTorch's GEMMs on the same device, i.e. the ROCm
+libraries it dispatches to; timed with torch events, best of 5 rounds of 20
+calls, and printed as TFLOP/s; comparison only, nothing of this library
+runs, and they take a process of their own):
+  vendor_sgemm8192  torch.addmm, FP32, C = 1.5*A*B - 0.5*C, 8192^3
+  vendor_sgemm2048  the same at 2048^3;  vendor_sgemm1024 at 1024^3
+  vendor_dgemm8192  the same in FP64;    vendor_dgemm1024 at 1024^3
+  vendor_bf16_8192  torch.matmul of bf16 operands (bf16 output), 8192^3
+  vendor_fp8_8192   torch._scaled_mm, e4m3 operands, unit tensor scales,
+                    FP32 output, 8192^3 (vendor_fp8_8192_bf16out: bf16 out)
 """
 import argparse
 import sys
@@ -56,6 +67,50 @@ def install_tile(ctx, name, tile):
     dt, ta, tb, M, N, K = _GEMM_SHAPES[name]
     _capi.check(ctx.lib.gat_tuned_set(ctx.h, dt, ta, tb, M, N, K, ctypes.byref(_capi.TileCfg(*tile))),
                 "gat_tuned_set")
+
+
+def vendor(name):
+    """TFLOP/s of PyTorch's own GEMM for one vendor_* workload."""
+    import torch
+    dev = "cuda:0"
+    g = torch.Generator(device=dev).manual_seed(0x6A7A5EED)
+
+    def uni(n, dt):
+        return (torch.rand(n, n, device=dev, generator=g, dtype=torch.float32) * 2 - 1).to(dt)
+
+    import re
+    kind = name.split("_")[1]
+    n = int(re.search(r"\d{3,}", name).group(0))  # the size: the first run of 3+ digits
+    if kind.startswith("sgemm") or kind.startswith("dgemm"):
+        dt = torch.float32 if kind[0] == "s" else torch.float64
+        A, B, C = uni(n, dt), uni(n, dt), uni(n, dt)
+        fn = lambda: torch.addmm(C, A, B, beta=-0.5, alpha=1.5)  # noqa: E731
+    elif kind == "bf16":
+        A, B = uni(n, torch.bfloat16), uni(n, torch.bfloat16)
+        fn = lambda: A @ B  # noqa: E731
+    elif kind == "fp8":
+        A8 = uni(n, torch.float32).to(torch.float8_e4m3fn)
+        B8 = uni(n, torch.float32).to(torch.float8_e4m3fn).t().contiguous().t()  # column-major operand
+        one = torch.tensor(1.0, device=dev)
+        out = torch.bfloat16 if name.endswith("bf16out") else torch.float32
+        fn = lambda: torch._scaled_mm(A8, B8, scale_a=one, scale_b=one, out_dtype=out)  # noqa: E731
+    else:
+        raise SystemExit("unknown workload %s" % name)
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = float("inf")
+    for _ in range(5):
+        s.record()
+        for _ in range(20):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        best = min(best, s.elapsed_time(e) / 20)
+    tf = 2.0 * n ** 3 / (best * 1e-3) / 1e12
+    print("[kernel_runs] %s %.4f ms %.1f TFLOP/s" % (name, best, tf), flush=True)
+    return tf
 
 
 def run(ctx, name, reps):
@@ -138,6 +193,15 @@ def main(argv=None):
     tile = None if a.tile is None else tuple(int(v) for v in a.tile.split(","))
     if tile is not None and len(tile) != 7:
         raise SystemExit("--tile takes 7 integers")
+    vend = [w.startswith("vendor_") for w in a.workloads]
+    if any(vend):
+        # PyTorch brings a HIP runtime of its own, which cannot open the
+        # device in a process where this library's runtime already has
+        if not all(vend):
+            raise SystemExit("vendor_* workloads run in a process of their own")
+        for w in a.workloads:
+            vendor(w)
+        return 0
     ctx = Context(0)
     for w in a.workloads:
         if tile is not None:

@@ -28,6 +28,11 @@ CASES = [
     (MATMUL, _capi.GAT_F32, 1, MFMA, [1, 1, 300, 260, 140, 0, 0, 2, 2, 2, 2, 2]),
     (MATMUL, _capi.GAT_F32, 1, MFMA, [3, 0, 256, 256, 128, 1, 1, 2, 2, 1, 1, 2]),
     (MATMUL, _capi.GAT_F64, 1, MFMA, [1, 1, 128, 96, 80, 1, 0, 2, 2, 2, 2, 5]),
+    # FP8 tiles are 128 deep: the depth digit has one value, so xp 0 is
+    # (depth 128, 32x32 blocks, no k-split); K = 200 runs a ragged last stage
+    (MATMUL, _capi.GAT_FP8, 1, MFMA, [1, 0, 256, 384, 512, 0, 1, 2, 2, 2, 2, 0]),
+    (MATMUL, _capi.GAT_FP8, 1, MFMA, [1, 1, 200, 130, 200, 1, 0, 2, 4, 2, 2, 0]),
+    (MATMUL, _capi.GAT_BF16, 1, MFMA, [1, 1, 192, 160, 96, 0, 1, 2, 2, 2, 2, 3]),
     (MATVEC, _capi.GAT_F32, 4, BUFFER, [1, 1, 1024, 512, 0, 64, 8, 4, 1]),
     (MATVEC, _capi.GAT_F64, 2, IMAGE, [1, 0, 512, 512, 1, 32, 4, 2, 3]),
     (MATVEC, _capi.GAT_F32, 4, NATIVE, [1, 1, 1000, 1028, 0, 256, 4, 4, 1]),
@@ -102,6 +107,15 @@ def test_bench_matmul_cli_em_mfma_and_replay(tmp_path):
     rc, out2, err2 = _run(cmd)  # every trial is journaled: replay only
     assert rc == 0 and "***DONE***" in out2, err2
     assert open(j).read().count("\n") == lines1
+
+
+def test_bench_matmul_cli_em_fp8_and_bf16(tmp_path):
+    # every matrix-core form the tool names reaches a tuned point
+    for t in ("fp8mfma", "bf16mfma"):
+        j = str(tmp_path / (t + ".jnl"))
+        rc, out, err = _run([os.path.join(BIN, "bench_matmul"), "-d", "gpu", "-j", j, "-T", t, "-n", "1024", "-e"])
+        assert rc == 0 and "***DONE***" in out and "no good kernels" not in out + err, (t, out, err)
+        assert "\t-4" in open(j).read(), t
 
 
 def test_bench_matvec_and_saxpy_cli_em(tmp_path):
