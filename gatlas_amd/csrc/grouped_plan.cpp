@@ -67,11 +67,11 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
     probs.resize(n);
     shapes.assign(n, 0);
     flops = 0;
-    // A unit is a band of tile rows of one problem: a problem of few tiles is
-    // one unit, a larger one is cut into up to GP_XCDS bands.
+    // A unit is a block of tiles of one problem: a problem of few tiles is
+    // one unit, a larger one is cut into up to GP_XCDS blocks.
     struct Unit {
         double cost, kcost;
-        int p, r0, r1;
+        int p, r0, r1, c0, c1;  // tile rows [r0, r1) x tile columns [c0, c1)
     };
     std::vector<Unit> units;
     // per problem: (offset, extent) of its tile rows and columns
@@ -114,15 +114,37 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
         tile_spans(g.N, gp_shape_bn(s), cols.back());
         const int tm = (int)rows.back().size(), tn = (int)cols.back().size();
         const double kc = (double)(g.K + 32);  // cost model: tile area times (K + 32)
-        const int nb = (tm * tn <= 32) ? 1 : std::min(GP_XCDS, tm);
-        for (int b = 0; b < nb && tm > 0 && tn > 0; ++b) {
-            const int r0 = b * tm / nb, r1 = (b + 1) * tm / nb;
-            if (r1 <= r0) continue;
-            double area = 0;
-            for (int r = r0; r < r1; ++r)
-                for (const auto& c : cols.back()) area += (double)rows.back()[r].second * c.second;
-            units.push_back(Unit{area * kc, kc, (int)i, r0, r1});
+        // A problem of many tiles is cut into up to GP_XCDS blocks of tiles
+        // (one per 32 tiles), bm tile rows by bn tile columns: each block
+        // reads its A panels once per block column and its B panels once
+        // per block row, so of the cuts into that many blocks the one with
+        // the fewest panel bytes is taken (a cut into row bands only would
+        // have every band read all of B).
+        int bm = 1, bn = 1;
+        if (tm > 0 && tn > 0) {
+            const int want = std::min(GP_XCDS, std::max(1, tm * tn / 32));
+            double best = -1;
+            for (int a = 1; a <= want; ++a)
+                for (int b = 1; a * b <= want; ++b) {
+                    if (a > tm || b > tn || a * b != want) continue;
+                    const double bytes = (double)g.M * b + (double)g.N * a;
+                    if (best < 0 || bytes < best) {
+                        best = bytes;
+                        bm = a;
+                        bn = b;
+                    }
+                }
         }
+        for (int bi = 0; bi < bm && tm > 0 && tn > 0; ++bi)
+            for (int bj = 0; bj < bn; ++bj) {
+                const int r0 = bi * tm / bm, r1 = (bi + 1) * tm / bm;
+                const int c0 = bj * tn / bn, c1 = (bj + 1) * tn / bn;
+                if (r1 <= r0 || c1 <= c0) continue;
+                double area = 0;
+                for (int r = r0; r < r1; ++r)
+                    for (int c = c0; c < c1; ++c) area += (double)rows.back()[r].second * cols.back()[c].second;
+                units.push_back(Unit{area * kc, kc, (int)i, r0, r1, c0, c1});
+            }
         flops += 2.0 * (double)g.M * (double)g.N * (double)g.K;
     }
     // Units onto XCD queues, longest first onto the least-loaded queue, so
@@ -139,10 +161,11 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
         for (int y = 1; y < GP_XCDS; ++y)
             if (load[y] < load[x]) x = y;
         load[x] += u.cost;
-        // column-major inside the band: consecutive tiles share a B panel
-        for (const auto& c : cols[u.p])
+        // column-major inside the block: consecutive tiles share a B panel
+        for (int ci = u.c0; ci < u.c1; ++ci)
             for (int r = u.r0; r < u.r1; ++r) {
                 const auto& rw = rows[u.p][r];
+                const auto& c = cols[u.p][ci];
                 lists[x].push_back(Tile{(double)rw.second * c.second * u.kcost, u.p, rw.first, c.first,
                                         gp_shape_of(rw.second, c.second)});
             }

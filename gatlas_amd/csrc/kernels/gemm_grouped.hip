@@ -2,7 +2,7 @@
 // and transposes in one launch.
 //
 // The host splits the (problem, m0, n0) tiles into one queue per XCD: each
-// problem, or each band of tile rows of a large one, goes whole to one queue
+// problem, or each block of tile rows and columns of a large one, goes whole to one queue
 // (longest first onto the least-loaded queue), so its operand panels are
 // read into one XCD's L2; inside a queue the longest tiles start first.
 // A grid of about two work-groups per CU loops: one work-item takes the
