@@ -7,7 +7,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgatlas_hip.so")
+# GATLAS_LIB names another build of the same library (e.g. the host-sanitised
+# one the CPU tests run against); by default the in-tree build
+LIB_PATH = os.environ.get("GATLAS_LIB") or os.path.join(_HERE, "libgatlas_hip.so")
 
 GAT_F32, GAT_F64, GAT_BF16, GAT_BF16X3, GAT_FP8 = 0, 1, 2, 3, 4
 GAT_MATMUL, GAT_MATVEC, GAT_SAXPY = 0, 1, 2
