@@ -130,7 +130,7 @@ struct DmaF32Operand;
 template <int ROWS, int NW, int SW>
 struct DmaF32Operand<ROWS, true, NW, SW> {
     typedef DmaColsF32<ROWS, NW, SW> Op;
-    static constexpr int BYTES = Op::BYTES;
+    static constexpr int BYTES = Op::BYTES, PER_WAVE = Op::PER_WAVE;
     __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
                                                  int lane) {
         Op::issue(src, ld, r0, k0, lds, w, lane);
@@ -147,7 +147,7 @@ struct DmaF32Operand<ROWS, true, NW, SW> {
 template <int ROWS, int NW, int SW>
 struct DmaF32Operand<ROWS, false, NW, SW> {
     typedef DmaRowsF32<ROWS, NW> Op;
-    static constexpr int BYTES = Op::BYTES;
+    static constexpr int BYTES = Op::BYTES, PER_WAVE = Op::PER_WAVE;
     __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
                                                  int lane) {
         Op::issue(src, ld, r0, k0, lds, w, lane);
@@ -244,29 +244,39 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
         }
     };
 
+    constexpr int LOADS = OpA::PER_WAVE + OpB::PER_WAVE;  // loads per wave per stage
+    auto issue = [&](int st, char* buf) {
+        OpA::issue(A, lda, m0, kbase(st), buf, w, lane);
+        OpB::issue(B, ldb, n0, kbase(st), buf + OpA::BYTES, w, lane);
+    };
     if (kh) {
         OpA::stage_head(A, lda, m0, 0, kh, sb, threadIdx.x, NW * 64);
         OpB::stage_head(B, ldb, n0, 0, kh, sb + OpA::BYTES, threadIdx.x, NW * 64);
     } else {
-        OpA::issue(A, lda, m0, 0, sb, w, lane);
-        OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
+        issue(0, sb);
     }
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
+    if (KT > 1) {
+        issue(1, sb + STAGE);
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(LOADS, 0));  // stage 0 in LDS
+    } else {
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));
+    }
+    __builtin_amdgcn_s_barrier();
     frags(sb, kb, a0, b0);
 
     // Software pipeline over the wave's NS 8-deep steps of a stage: the
-    // fragments of step s+1 are read while the MFMAs of step s run; the wait
-    // for the next stage and the barrier come before the last step, whose
-    // MFMAs run under the reads of the next stage's first fragments.  The
-    // next stage is fetched unconditionally (the last iteration fetches its
-    // own stage again into the idle buffer).
+    // fragments of step s+1 are read while the MFMAs of step s run.  Before
+    // the last step each wave waits for the next stage and for its own
+    // reads of this one, and the waves meet at a barrier; this stage's
+    // buffer is then free, and the loads of the stage after next go into it
+    // while the last step's MFMAs run under the reads of the next stage's
+    // first fragments: a stage's loads have a whole iteration to land.  Past
+    // the last stage the loads repeat it (unused), so every iteration has
+    // the same shape.
     for (int t = 0; t < KT; ++t) {
-        const char* cur = sb + (t & 1) * STAGE;
-        char* nxt = sb + ((t + 1) & 1) * STAGE;
-        const int kn = kbase(t + 1 < KT ? t + 1 : t);  // K >= 32: the last stage is whole
-        OpA::issue(A, lda, m0, kn, nxt, w, lane);
-        OpB::issue(B, ldb, n0, kn, nxt + OpA::BYTES, w, lane);
+        char* cur = sb + (t & 1) * STAGE;
+        const char* nxt = sb + ((t + 1) & 1) * STAGE;
+        __builtin_amdgcn_sched_barrier(0);
         if constexpr (NS == 4) {
             frags(cur, kb + 8, a1, b1);
             mma(a0, b0);
@@ -282,12 +292,17 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
             mma(a0, b0);
             interleave();
         }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
-        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the next stage in; this wave's reads of this one done
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        issue(t + 2 < KT ? t + 2 : KT - 1, cur);
         frags(nxt, kb, a0, b0);
         mma(a1, b1);
         interleave();
+        __builtin_amdgcn_sched_barrier(0);
     }
+    __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the repeated loads land before LDS is reused
     // the k-split exchange reuses the stage buffers: every wave's last
     // fragment reads must be done
     if constexpr (WK > 1) __syncthreads();
