@@ -120,29 +120,44 @@ __global__ __launch_bounds__(WM* WN * 64) __attribute__((amdgpu_waves_per_eu(2))
     };
 
     const int KT = kpad / 32;
-    OpA::issue(As, ld2, m0, 0, sb, w, lane);
-    OpB::issue(Bs, ld2, n0, 0, sb + OpA::BYTES, w, lane);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
+    constexpr int LOADS = OpA::PER_WAVE + OpB::PER_WAVE;  // loads per wave per stage
+    auto issue = [&](int t, char* buf) {
+        OpA::issue(As, ld2, m0, 64 * t, buf, w, lane);
+        OpB::issue(Bs, ld2, n0, 64 * t, buf + OpA::BYTES, w, lane);
+    };
+    issue(0, sb);
+    if (KT > 1) {
+        issue(1, sb + STAGE);
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(LOADS, 0));  // stage 0 in LDS
+    } else {
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));
+    }
+    __builtin_amdgcn_s_barrier();
     frags(sb, 0, f0);
-    // The fragments of step 1 are read under the MFMAs of step 0; the wait
-    // for the next stage and the barrier come before step 1's MFMAs, which
-    // run under the reads of the next stage's step 0.
+    // The fragments of step 1 are read under the MFMAs of step 0.  Then each
+    // wave waits for the next stage and its own reads of this one, and the
+    // waves meet at a barrier; this stage's buffer is free, and the loads of
+    // the stage after next go into it while step 1's MFMAs run under the
+    // reads of the next stage's step 0, so a stage's loads have a whole
+    // iteration to land.  Past the last stage the loads repeat it (unused).
     for (int t = 0; t < KT; ++t) {
-        const char* cur = sb + (t & 1) * STAGE;
-        char* nxt = sb + ((t + 1) & 1) * STAGE;
-        const int tn = t + 1 < KT ? t + 1 : t;  // the last iteration fetches its own stage again
-        OpA::issue(As, ld2, m0, 64 * tn, nxt, w, lane);
-        OpB::issue(Bs, ld2, n0, 64 * tn, nxt + OpA::BYTES, w, lane);
+        char* cur = sb + (t & 1) * STAGE;
+        const char* nxt = sb + ((t + 1) & 1) * STAGE;
+        __builtin_amdgcn_sched_barrier(0);
         frags(cur, 1, f1);
         mma3(f0);
         interleave();
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
-        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the next stage in; this wave's reads of this one done
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        issue(t + 2 < KT ? t + 2 : KT - 1, cur);
         frags(nxt, 0, f0);
         mma3(f1);
         interleave();
+        __builtin_amdgcn_sched_barrier(0);
     }
+    __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the repeated loads land before the wave ends
 
 #pragma unroll
     for (int i = 0; i < TM; ++i)
