@@ -30,11 +30,6 @@ typedef __attribute__((address_space(1))) void glb_void_t;
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) s16x4 lds_s16x4_t;
 
-// s_waitcnt operand for gfx9-family targets: vmcnt in bits 3:0 and 15:14,
-// expcnt (7: no wait) in 6:4, lgkmcnt in 11:8.  Issued as a builtin, the
-// wait is known to the compiler, which then adds none of its own after it.
-constexpr int waitcnt_imm(int vm, int lgkm) { return (vm & 15) | ((vm >> 4) << 14) | (7 << 4) | (lgkm << 8); }
-
 // One operand of a K stage: ROWS rows of 128 bytes of k (64 bf16 or 128
 // fp8 values), loaded by NW wavefronts; T is the element type.  The
 // row-contiguous layout exists for 2-byte elements only.

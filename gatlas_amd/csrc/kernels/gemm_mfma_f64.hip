@@ -12,6 +12,7 @@
 // loop, and group g adds the WK partials of accumulator register g*4/WK..
 // in group order and stores those outputs.
 #include "../kernels.hpp"
+#include "mfma_common.hpp"
 
 namespace gat {
 
@@ -290,38 +291,54 @@ __device__ __forceinline__ void dgemm_tile_dma(const double* __restrict__ A, con
         }
     };
 
-    OpA::issue(A, lda, m0, 0, sb, w, lane);
-    OpB::issue(B, ldb, n0, 0, sb + OpA::BYTES, w, lane);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
+    constexpr int LOADS = OpA::PER_WAVE + OpB::PER_WAVE;  // loads per wave per stage
+    auto issue = [&](int t, char* buf) {
+        OpA::issue(A, lda, m0, 16 * t, buf, w, lane);
+        OpB::issue(B, ldb, n0, 16 * t, buf + OpA::BYTES, w, lane);
+    };
+    issue(0, sb);
+    if (KT > 1) {
+        issue(1, sb + STAGE);
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(LOADS, 0));  // stage 0 in LDS
+    } else {
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));
+    }
+    __builtin_amdgcn_s_barrier();
     frags(sb, kb, a0, b0);
+    // Before a stage's last step each wave waits for the next stage and its
+    // own reads of this one, and the waves meet at a barrier; this stage's
+    // buffer is then free, and the loads of the stage after next go into it
+    // under the last step's MFMAs (a whole iteration to land).  Past the
+    // last stage the loads repeat it (unused).
     for (int t = 0; t < KT; ++t) {
-        const char* cur = sb + (t & 1) * STAGE;
-        char* nxt = sb + ((t + 1) & 1) * STAGE;
-        const int kn = (t + 1 < KT ? t + 1 : t) * 16;
-        OpA::issue(A, lda, m0, kn, nxt, w, lane);
-        OpB::issue(B, ldb, n0, kn, nxt + OpA::BYTES, w, lane);
+        char* cur = sb + (t & 1) * STAGE;
+        const char* nxt = sb + ((t + 1) & 1) * STAGE;
+        __builtin_amdgcn_sched_barrier(0);
         if constexpr (NS == 2) {
             frags(cur, kb + 8, a1, b1);
             mma(a0, b0);
             interleave();
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next stage is in LDS
-            __syncthreads();
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the next stage in; this wave's reads of this one done
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_sched_barrier(0);
+            issue(t + 2 < KT ? t + 2 : KT - 1, cur);
             frags(nxt, kb, a0, b0);
             mma(a1, b1);
             interleave();
         } else {
             mma(a0, b0);
             interleave();
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            __syncthreads();
-            frags(nxt, kb, a1, b1);
-#pragma unroll
-            for (int i = 0; i < TM; ++i) a0[i] = a1[i];
-#pragma unroll
-            for (int j = 0; j < TN; ++j) b0[j] = b1[j];
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_sched_barrier(0);
+            issue(t + 2 < KT ? t + 2 : KT - 1, cur);
+            frags(nxt, kb, a0, b0);
         }
+        __builtin_amdgcn_sched_barrier(0);
     }
+    __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the repeated loads land before LDS is reused
     if constexpr (WK > 1) __syncthreads();  // the exchange reuses the stage buffers
     dgemm_finish<WM, WN, TM, TN, WK, false>(acc, C, M, N, ldc, alpha, beta, m0, n0, wm, wn, wmn, wk, lane, smemd);
 }

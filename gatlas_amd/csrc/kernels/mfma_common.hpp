@@ -6,6 +6,11 @@
 
 namespace gat {
 
+// s_waitcnt operand for gfx9-family targets: vmcnt in bits 3:0 and 15:14,
+// expcnt (7: no wait) in 6:4, lgkmcnt in 11:8.  Issued as a builtin, the
+// wait is known to the compiler, which then adds none of its own after it.
+constexpr int waitcnt_imm(int vm, int lgkm) { return (vm & 15) | ((vm >> 4) << 14) | (7 << 4) | (lgkm << 8); }
+
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
