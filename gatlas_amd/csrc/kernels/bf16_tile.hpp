@@ -28,6 +28,17 @@ __device__ __forceinline__ void split_bf16(float a, __bf16& hi, __bf16& lo) {
     lo = (__bf16)(a - (float)hi);
 }
 
+// split_bf16 of two elements, bit patterns packed first-in-low-half: one
+// paired conversion per plane, and the hi values widened back to FP32 by a
+// shift (first) and a mask (second) instead of a conversion each.
+__device__ __forceinline__ void split_bf16x2(float a0, float a1, unsigned& hi, unsigned& lo) {
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2v __attribute__((ext_vector_type(2)));
+    hi = __builtin_bit_cast(unsigned, __builtin_convertvector((f2v){a0, a1}, b2v));
+    const float h0 = __builtin_bit_cast(float, hi << 16), h1 = __builtin_bit_cast(float, hi & 0xFFFF0000u);
+    lo = __builtin_bit_cast(unsigned, __builtin_convertvector((f2v){a0 - h0, a1 - h1}, b2v));
+}
+
 // Global -> register -> LDS staging of a ROWS x BK tile for one operand.
 //   direct (k-contiguous source): element (r, k) at src[(r0 + r) * ld + k0 + k]
 //   transposed (row-contiguous):  element (r, k) at src[(k0 + k) * ld + r0 + r]
@@ -66,6 +77,18 @@ __device__ __forceinline__ void store_kvec(__bf16* dst, const unsigned (&b)[KB])
 }
 
 __device__ __forceinline__ unsigned bf16_bits(__bf16 h) { return (unsigned)__builtin_bit_cast(unsigned short, h); }
+
+// Writes NW packed dwords (two bf16 each) to dst as one 4-, 8- or 16-byte store
+template <int NW>
+__device__ __forceinline__ void store_words(__bf16* dst, const unsigned (&w)[NW]) {
+    if constexpr (NW == 1) {
+        *(unsigned*)dst = w[0];
+    } else if constexpr (NW == 2) {
+        *(u32x2*)dst = u32x2{w[0], w[1]};
+    } else {
+        *(u32x4*)dst = u32x4{w[0], w[1], w[2], w[3]};
+    }
+}
 
 template <int MODE, int ROWS, int BK, int NT, bool TRANS>
 struct StageIO;
@@ -227,11 +250,11 @@ struct StageIO<MODE_X3, ROWS, BK, NT, TRANS> {
             for (int q = 0; q < NV; ++q) {
                 int r, k;
                 coords(q, tid, r, k);
-                __bf16 h[W], l[W];
-#pragma unroll
-                for (int e = 0; e < W; ++e) split_bf16(reg[q][e], h[e], l[e]);
-                *(u32x2*)(hi + r * LDK + k) = *(const u32x2*)h;
-                *(u32x2*)(lo + r * LDK + k) = *(const u32x2*)l;
+                unsigned h0, l0, h1, l1;
+                split_bf16x2(reg[q][0], reg[q][1], h0, l0);
+                split_bf16x2(reg[q][2], reg[q][3], h1, l1);
+                *(u32x2*)(hi + r * LDK + k) = u32x2{h0, h1};
+                *(u32x2*)(lo + r * LDK + k) = u32x2{l0, l1};
             }
         } else {
 #pragma unroll
@@ -240,16 +263,20 @@ struct StageIO<MODE_X3, ROWS, BK, NT, TRANS> {
                 coords(qb, tid, r, k);
 #pragma unroll
                 for (int e = 0; e < W; ++e) {
-                    unsigned bh[KB], bl[KB];
-#pragma unroll
-                    for (int j = 0; j < KB; ++j) {
+                    if constexpr (KB == 1) {
                         __bf16 h, l;
-                        split_bf16(reg[qb + j][e], h, l);
-                        bh[j] = bf16_bits(h);
-                        bl[j] = bf16_bits(l);
+                        split_bf16(reg[qb][e], h, l);
+                        *(unsigned short*)(hi + (r + e) * LDK + k) = (unsigned short)bf16_bits(h);
+                        *(unsigned short*)(lo + (r + e) * LDK + k) = (unsigned short)bf16_bits(l);
+                    } else {
+                        // k pairs (j, j + 1) split together, packed first-in-low-half
+                        unsigned wh[KB / 2], wl[KB / 2];
+#pragma unroll
+                        for (int j = 0; j < KB; j += 2)
+                            split_bf16x2(reg[qb + j][e], reg[qb + j + 1][e], wh[j / 2], wl[j / 2]);
+                        store_words<KB / 2>(hi + (r + e) * LDK + k, wh);
+                        store_words<KB / 2>(lo + (r + e) * LDK + k, wl);
                     }
-                    store_kvec<KB>(hi + (r + e) * LDK + k, bh);
-                    store_kvec<KB>(lo + (r + e) * LDK + k, bl);
                 }
             }
         }

@@ -20,21 +20,22 @@
 // blocks), 1 = 64 x 64 (one block per wave), 2 = 128 x 64 and 3 = 64 x 128
 // (2 x 1 and 1 x 2 blocks per wave); each work entry carries its tile's.
 // BF16 interior tiles of problems with K >= 64 a multiple of 8, and FP32
-// interior tiles of problems with K >= 32 a multiple of 16, with 16-byte aligned
-// rows, stage straight into LDS (bf16_dma_tile.hpp, sgemm_dma_tile.hpp);
-// the others, and every split-BF16x3 tile, stage through registers.  The
-// split-BF16x3 form splits FP32 operands into bf16 hi/lo planes while a
-// stage is stored to LDS (bf16_tile.hpp, MODE_X3): the routine of the
-// single-problem x3 kernel of that tile shape, staged 16 (128 x 128) or 32
-// (64 x 64) deep so that three groups fit a CU (the 128 x 64 shapes 16
-// deep).  The staging depth does not
-// change an x3 tile's bits (its products run in 16-deep steps in k order
-// at any depth), so a problem's result equals a single launch's.
+// and split-BF16x3 interior tiles of problems with K >= 32 a multiple of
+// 16, with 16-byte aligned rows, stage straight into LDS
+// (bf16_dma_tile.hpp, sgemm_dma_tile.hpp, x3_dma_tile.hpp); the others
+// stage through registers.  The split-BF16x3 form splits FP32 operands into
+// bf16 hi/lo as each wave reads its fragments (direct-to-LDS tiles) or while
+// a stage is stored to LDS (bf16_tile.hpp, MODE_X3, staged 16 deep for the
+// 128-wide shapes, 32 for 64 x 64); two groups fit a CU.  Neither the
+// staging path nor its depth changes an x3 tile's bits (the same split, and
+// its products run in 16-deep steps in k order), so a problem's result
+// equals a single launch's.
 #include "bf16_dma_tile.hpp"
 #include "bf16_tile.hpp"
 #include "grouped.hpp"
 #include "sgemm_dma_tile.hpp"
 #include "sgemm_tile.hpp"
+#include "x3_dma_tile.hpp"
 
 namespace gat {
 
@@ -89,10 +90,20 @@ __device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int 
     else bf16_tile_t<1, 2, 32, TA, TB>(p, m0, n0, smem);
 }
 
+// Split-BF16x3 interior tiles of problems with K >= 32 a multiple of 16 and
+// 16-byte aligned rows stage the FP32 operands straight into LDS and split
+// them as the fragments are read (x3_dma_tile.hpp, the same bits); the
+// others split while staging through registers, BK deep.
 template <int TM, int TN, int BK, bool TA, bool TB>
 __device__ __forceinline__ void x3_tile_t(const GroupedProblemDev& p, int m0, int n0, __bf16* smem) {
-    bf16_tile<2, 2, TM, TN, BK, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
-                                                 m0, n0, (p.flags & GP_VEC) != 0, smem);
+    constexpr int BM = 2 * TM * 32, BN = 2 * TN * 32;
+    const bool vec = (p.flags & GP_VEC) != 0;
+    if (vec && p.K >= 32 && p.K % 16 == 0 && m0 + BM <= p.M && n0 + BN <= p.N)
+        x3_tile_dma<2, 2, TM, TN, TA, TB>((const float*)p.A, (const float*)p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb,
+                                          p.ldc, p.alpha, p.beta, m0, n0, (float*)smem);
+    else
+        bf16_tile<2, 2, TM, TN, BK, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+                                                     p.beta, m0, n0, vec, smem);
 }
 
 template <bool TA, bool TB>
@@ -126,7 +137,7 @@ constexpr size_t grouped_tile_bytes() {
     // direct-to-LDS stages (the 128 x 128 shape's are the largest)
     constexpr size_t dma = FORM == GP_FORM_BF16 ? Bf16DmaShape<2, 2, 2, 2>::SMEM
                            : FORM == GP_FORM_F32 ? SgemmDmaShape<2, 2, 2, 2>::SMEM
-                                                 : 0;
+                                                 : X3DmaShape<2, 2, 2, 2>::SMEM;
     return (gp_max(reg, dma) + 15) / 16 * 16;
 }
 

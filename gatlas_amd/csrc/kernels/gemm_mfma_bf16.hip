@@ -1,10 +1,25 @@
 // BF16 matrix-core GEMMs on gfx950: one work-group per macro tile in
 // XCD-aware supertile order; the tile computation is bf16_tile
-// (bf16_tile.hpp), shared with the persistent grouped kernel.
+// (bf16_tile.hpp), shared with the persistent grouped kernel.  Interior
+// tiles stage straight into LDS where a direct-to-LDS routine exists: BF16
+// inputs at BK = 64 (bf16_dma_tile.hpp), split-BF16x3 at any BK
+// (x3_dma_tile.hpp, 32-deep stages); both give the register path's bits.
 #include "bf16_dma_tile.hpp"
 #include "bf16_tile.hpp"
+#include "x3_dma_tile.hpp"
 
 namespace gat {
+
+// LDS of a configuration: its register-staged stages or its direct-to-LDS
+// stages, whichever is larger.
+template <int WM, int WN, int TM, int TN, int BK, int MODE>
+constexpr size_t bf16_smem() {
+    constexpr size_t reg = Bf16Shape<WM, WN, TM, TN, BK, MODE>::SMEM;
+    constexpr size_t dma = MODE == MODE_BF16 ? (BK == 64 ? Bf16DmaShape<WM, WN, TM, TN>::SMEM : 0)
+                           : x3_has_dma<WM, WN, TM, TN>() ? X3DmaShape<WM, WN, TM, TN>::SMEM
+                                                           : 0;
+    return reg > dma ? reg : dma;
+}
 
 // Waves per SIMD the kernel asks the register allocator to allow: what the
 // LDS footprint permits (160 KiB per CU), capped by what the 512 registers
@@ -14,7 +29,7 @@ namespace gat {
 // full VGPR set and leave one wave per SIMD where LDS allows two.
 template <int WM, int WN, int TM, int TN, int BK, int MODE>
 constexpr int bf16_waves_per_simd() {
-    constexpr int groups = (int)(163840 / Bf16Shape<WM, WN, TM, TN, BK, MODE>::SMEM);
+    constexpr int groups = (int)(163840 / bf16_smem<WM, WN, TM, TN, BK, MODE>());
     constexpr int by_lds = groups * WM * WN / 4;
     constexpr int by_regs = 512 / (TM * TN * 16 + (MODE == MODE_X3 ? 160 : 96));
     constexpr int w = by_lds < by_regs ? by_lds : by_regs;
@@ -41,6 +56,15 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_simd<WM, WN, TM, TN, BK, MODE>
         if (vec_ok && K >= 64 && K % 8 == 0 && m0 + S::BM <= M && n0 + S::BN <= N) {
             bf16_tile_dma<WM, WN, TM, TN, TA, TB>((const __bf16*)A, (const __bf16*)B, C, M, N, K, lda, ldb, ldc,
                                                   alpha, beta, m0, n0, smemh);
+            return;
+        }
+    }
+    if constexpr (MODE == MODE_X3 && x3_has_dma<WM, WN, TM, TN>()) {
+        // interior tile, K >= 32 a multiple of 16, 16-byte aligned rows
+        const int m0 = tmi * S::BM, n0 = tni * S::BN;
+        if (vec_ok && K >= 32 && K % 16 == 0 && m0 + S::BM <= M && n0 + S::BN <= N) {
+            x3_tile_dma<WM, WN, TM, TN, TA, TB>((const float*)A, (const float*)B, C, M, N, K, lda, ldb, ldc, alpha,
+                                                beta, m0, n0, (float*)smemh);
             return;
         }
     }
@@ -104,10 +128,7 @@ bool mfma_bf16_has(int dtype, const MfmaCfg& c) {
 template <int WM, int WN, int TM, int TN, int BK, int MODE>
 static int launch_bf16_one(const GemmArgs& a, hipStream_t st) {
     constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
-    constexpr int NPL = MODE == MODE_X3 ? 2 : 1;
-    constexpr size_t smem_reg = 2 * (size_t)NPL * (BM + BN) * (BK + 8) * sizeof(__bf16);
-    constexpr size_t smem_dma = (MODE == MODE_BF16 && BK == 64) ? Bf16DmaShape<WM, WN, TM, TN>::SMEM : 0;
-    constexpr size_t smem = smem_reg > smem_dma ? smem_reg : smem_dma;
+    constexpr size_t smem = bf16_smem<WM, WN, TM, TN, BK, MODE>();
     const int tiles_m = (int)((a.M + BM - 1) / BM), tiles_n = (int)((a.N + BN - 1) / BN);
     const int W = MODE == MODE_X3 ? 4 : 8;  // elements per 16-byte vector
     const int vec_ok = ((uintptr_t)a.A % 16 == 0) && ((uintptr_t)a.B % 16 == 0) && (a.lda % W == 0) &&

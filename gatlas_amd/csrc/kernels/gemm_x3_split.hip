@@ -2,7 +2,7 @@
 //
 // A bandwidth pass splits op(A) (M x K) and op(B)^T (N x K) -- each element
 // a into hi = bf16(a) and lo = bf16(a - hi), the rounding of the in-kernel
-// form (split_bf16) -- into bf16 planes laid out k-contiguous, every 32 k
+// form (split_bf16x2) -- into bf16 planes laid out k-contiguous, every 32 k
 // values stored as [32 hi | 32 lo]: 128 bytes per row per 32-deep K stage.
 // Rows are padded with zeros to whole tiles and k to a multiple of 32, so
 // every tile of the GEMM is interior.  The GEMM stages both planes straight
@@ -34,10 +34,9 @@ __global__ __launch_bounds__(256) void x3_split_kernel(const float* __restrict__
     unsigned wv[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        __bf16 h0, l0, h1, l1;
-        split_bf16(tile[r][kb + 2 * j], h0, l0);
-        split_bf16(tile[r][kb + 2 * j + 1], h1, l1);
-        wv[j] = p < 2 ? pack_bf16(bf16_bits(h0), bf16_bits(h1)) : pack_bf16(bf16_bits(l0), bf16_bits(l1));
+        unsigned hw, lw;
+        split_bf16x2(tile[r][kb + 2 * j], tile[r][kb + 2 * j + 1], hw, lw);
+        wv[j] = p < 2 ? hw : lw;
     }
     __bf16* d = dst + (int64_t)(r0 + r) * ldd + b * 64 + (p < 2 ? 0 : 32) + kb;
     gstore(d, u32x4{wv[0], wv[1], wv[2], wv[3]});

@@ -30,7 +30,10 @@
 
 namespace gat {
 
-template <int ROWS, int NW>
+// SB selects the k bit that moves a k-row's chunks by 128 bytes: bit 2
+// for the FP32 fragments (k-rows kk + s and kk + 4 + s, read together),
+// bit 3 for the split-BF16x3 ones (k-rows kk + s and kk + 8 + s).
+template <int ROWS, int NW, int SB = 2>
 struct DmaRowsF32 {
     static constexpr int CPR = ROWS / 4;            // 16-byte chunks per k-row
     static constexpr int BYTES = 32 * ROWS * 4;     // one 32-deep stage
@@ -40,7 +43,7 @@ struct DmaRowsF32 {
     static_assert(PER_WAVE * NW == BLOCKS, "blocks must split evenly over the wavefronts");
     static_assert(CPR >= 16 && CPR <= 64 && 64 % CPR == 0, "row-contiguous FP32 tile: 64..256 rows");
 
-    __device__ static __forceinline__ int rsw(int k) { return 8 * ((k >> 2) & 1); }
+    __device__ static __forceinline__ int rsw(int k) { return 8 * ((k >> SB) & 1); }
 
     // Wavefront w fetches its blocks of the stage starting at k0 into `lds`.
     __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
@@ -53,6 +56,27 @@ struct DmaRowsF32 {
             __builtin_amdgcn_global_load_lds((glb_void_t*)g, (lds_void_t*)(lds + b * 1024), 16, 0, 0);
         }
     }
+
+    // The same loads through per-lane addresses computed once per tile:
+    // prep() gives the stage at k0 = 0, and the stage at k0 is then
+    // issue_at(p, k0 * ld), one 64-bit add per load.
+    __device__ static __forceinline__ void prep(const float* src, int64_t ld, int r0, int w, int lane,
+                                                const float* (&p)[PER_WAVE]) {
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int b = i * NW + w;
+            const int k = b * KPB + lane / CPR;
+            p[i] = src + (int64_t)k * ld + r0 + 4 * ((lane % CPR) ^ rsw(k));
+        }
+    }
+    __device__ static __forceinline__ void issue_at(const float* const (&p)[PER_WAVE], int64_t off, char* lds,
+                                                    int w) {
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i)
+            __builtin_amdgcn_global_load_lds((glb_void_t*)(p[i] + off), (lds_void_t*)(lds + (i * NW + w) * 1024), 16,
+                                             0, 0);
+    }
+    __device__ static __forceinline__ int64_t stage_off(int k0, int64_t ld) { return (int64_t)k0 * ld; }
 
     // The k values [k0, k0 + klen) (klen < 32 a multiple of 4) written into
     // the last klen k-rows of a stage in `lds`, the first 32 - klen zero,
@@ -77,6 +101,18 @@ struct DmaRowsF32 {
 #pragma unroll
         for (int s = 0; s < 4; ++s) v[s] = p[s * ROWS];
     }
+
+    // BF16 MFMA operand in FP32 (split-BF16x3 tiles, SB = 3): element
+    // (k = kk + 8*(lane/32) + e, row rbase + lane%32) in v[e], kk a multiple
+    // of 16, so all eight k-rows of a lane share the swizzle
+    __device__ static __forceinline__ void frag8(const char* lds, int rbase, int kk, int lane, float (&v)[8]) {
+        static_assert(SB == 3, "eight k-rows per lane need the bit-3 swizzle");
+        const int n = rbase + (lane & 31);
+        const int k = kk + 8 * (lane >> 5);
+        const float* p = (const float*)(lds + k * (ROWS * 4) + 16 * ((n >> 2) ^ rsw(k))) + (n & 3);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) v[s] = p[s * ROWS];
+    }
 };
 
 // k-contiguous FP32 operand of a 32-deep K stage: ROWS rows of 128 bytes.
@@ -100,6 +136,25 @@ struct DmaColsF32 {
         }
     }
 
+    // per-lane addresses computed once per tile (see DmaRowsF32::prep); the
+    // stage at k0 is issue_at(p, k0)
+    __device__ static __forceinline__ void prep(const float* src, int64_t ld, int r0, int w, int lane,
+                                                const float* (&p)[PER_WAVE]) {
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int row = 8 * (i * NW + w) + (lane >> 3);
+            p[i] = src + (int64_t)(r0 + row) * ld + 4 * ((lane & 7) ^ kswz(row));
+        }
+    }
+    __device__ static __forceinline__ void issue_at(const float* const (&p)[PER_WAVE], int64_t off, char* lds,
+                                                    int w) {
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i)
+            __builtin_amdgcn_global_load_lds((glb_void_t*)(p[i] + off), (lds_void_t*)(lds + (i * NW + w) * 1024), 16,
+                                             0, 0);
+    }
+    __device__ static __forceinline__ int64_t stage_off(int k0, int64_t /*ld*/) { return k0; }
+
     // short first stage through registers, k values in the last klen
     // positions of each row (see DmaRowsF32::stage_head)
     __device__ static __forceinline__ void stage_head(const float* src, int64_t ld, int r0, int k0, int klen,
@@ -121,6 +176,20 @@ struct DmaColsF32 {
         const f4v f = *(const f4v*)(lds + row * 128 + 16 * (c ^ kswz(row)));
 #pragma unroll
         for (int s = 0; s < 4; ++s) v[s] = f[s];
+    }
+
+    // BF16 MFMA operand in FP32 (split-BF16x3 tiles): element (row,
+    // k = kk + 8*(lane/32) + e) in v[e], kk a multiple of 16: two 16-byte reads
+    __device__ static __forceinline__ void frag8(const char* lds, int rbase, int kk, int lane, float (&v)[8]) {
+        const int row = rbase + (lane & 31);
+        const int c = (kk >> 2) + 2 * (lane >> 5);
+        const f4v f0 = *(const f4v*)(lds + row * 128 + 16 * (c ^ kswz(row)));
+        const f4v f1 = *(const f4v*)(lds + row * 128 + 16 * ((c + 1) ^ kswz(row)));
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            v[s] = f0[s];
+            v[s + 4] = f1[s];
+        }
     }
 };
 

@@ -158,6 +158,41 @@ def test_bf16_direct_lds_path(ctx, ta, tb, K):
             assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (cfg, ta, tb)
 
 
+@pytest.mark.parametrize("K", [96, 112, 100])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_bf16x3_direct_lds_path(ctx, ta, tb, K):
+    # Split-BF16x3 tiles of at most 2 x 2 blocks per wave stage interior
+    # tiles' FP32 operands straight from HBM into LDS and split each
+    # fragment as it is read; edge tiles split while staging through
+    # registers.  M, N give whole and partial tiles; K = 112 adds a short
+    # first stage of 16 k values staged through registers; K = 100 is not a
+    # multiple of 16, so every tile takes the register path.  Checked against
+    # the FP64 oracle, and bit for bit against the register path on the same
+    # data (leading dimensions one element longer, not 16-byte multiples,
+    # force that path on every tile).
+    M, N = 512, 384
+    A, B, C0 = _operands(M, N, K, ta, tb, 38)
+    ra, ca = (K, M) if ta else (M, K)
+    rb, cb = (N, K) if tb else (K, N)
+    dA, dB = ctx.to_device(A), ctx.to_device(B)
+    pA, pB = ctx.to_device(_pad_rows(A, ra, ca, 1)), ctx.to_device(_pad_rows(B, rb, cb, 1))
+    C64, MAG = ora.gemm64(A, B, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, C0=C0)
+    for cfg in ctx.tile_cfgs(_capi.GAT_BF16X3):
+        if tuple(cfg[:6]) == SPLIT_X3:
+            continue
+        _say("bf16x3 direct-LDS", cfg, ta, tb, K)
+        d1, d2 = ctx.to_device(C0), ctx.to_device(C0)
+        ctx.gemm(dA, dB, d1, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, dtype=_capi.GAT_BF16X3, cfg=cfg)
+        ctx.gemm(pA, pB, d2, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, dtype=_capi.GAT_BF16X3,
+                 lda=ca + 1, ldb=cb + 1, cfg=cfg)
+        got, ref = d1.download(), d2.download()
+        cw, _, _ = ora.error_metrics(got, C64, MAG)
+        assert cw <= TOL, (cfg, ta, tb, K, cw)
+        cw2, _, _ = ora.error_metrics(ref, C64, MAG)
+        assert cw2 <= TOL, (cfg, ta, tb, K, cw2)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (cfg, ta, tb, K)
+
+
 SPLIT_X3 = (2, 4, 4, 2, 32, 32)  # split-plane form of split-BF16x3
 REG_X3 = (2, 4, 4, 2, 16, 32)    # register-staged form, same tile
 
