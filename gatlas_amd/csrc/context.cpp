@@ -227,10 +227,11 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
         // 256 x 256, 8 waves; split-BF16x3 through pre-split planes
         big = dtype == DT_BF16 ? MfmaCfg{2, 4, 4, 2, 64, 32} : x3_split_cfg_value();
         // BF16: 128 x 128 staged 64 deep (interior tiles straight into LDS),
-        // 3072^3 673 TFLOP/s against 579 staged 32 deep.  Split-BF16x3
-        // keeps 32-deep stages: 64 deep ran 2048^3 at 201 against 188 but
-        // 3072^3 at 167 against 198.
-        mid = dtype == DT_BF16 ? MfmaCfg{2, 2, 2, 2, 64, 32} : MfmaCfg{2, 2, 2, 2, 32, 32};
+        // 3072^3 673 TFLOP/s against 579 staged 32 deep.  Split-BF16x3:
+        // interior tiles take the direct-to-LDS x3 routine at any BK; BK =
+        // 16 keeps the register-staged edge tiles' LDS at 48 KiB (2048^3
+        // 246-260 TFLOP/s against 212-233 for BK = 32).
+        mid = dtype == DT_BF16 ? MfmaCfg{2, 2, 2, 2, 64, 32} : MfmaCfg{2, 2, 2, 2, 16, 32};
         small = MfmaCfg{2, 2, 1, 1, 64, 32};
         tiny = MfmaCfg{1, 1, 2, 2, 64, 32};
     } else {

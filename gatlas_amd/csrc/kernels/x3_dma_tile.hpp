@@ -101,21 +101,25 @@ __device__ __forceinline__ void x3_tile_dma(const float* __restrict__ A, const f
     const int KT = K / 32 + (kh != 0);
     auto kbase = [&](int st) { return st == 0 ? 0 : st * 32 - (kh ? 32 - kh : 0); };  // first k of stage st
 
+    // Fragments of a 16-deep step go through three forms: FP32 values read
+    // from LDS, then bf16 hi/lo operands, then products.  A step's operands
+    // are split while the previous step's products run, and its FP32 values
+    // are read one step before that, so neither the LDS latency nor the
+    // split waits in front of the matrix instructions.
+    float ra0[TM][8], rb0[TN][8], ra1[TM][8], rb1[TN][8];
     bf16x8 ah0[TM], al0[TM], bh0[TN], bl0[TN], ah1[TM], al1[TM], bh1[TN], bl1[TN];
-    auto frags = [&](const char* stage, int kk, bf16x8(&ah)[TM], bf16x8(&al)[TM], bf16x8(&bh)[TN],
-                     bf16x8(&bl)[TN]) {
+    auto frags = [&](const char* stage, int kk, float(&ra)[TM][8], float(&rb)[TN][8]) {
 #pragma unroll
-        for (int i = 0; i < TM; ++i) {
-            float v[8];
-            OpA::frag8(stage, wm * TM * 32 + i * 32, kk, lane, v);
-            split_frag(v, ah[i], al[i]);
-        }
+        for (int i = 0; i < TM; ++i) OpA::frag8(stage, wm * TM * 32 + i * 32, kk, lane, ra[i]);
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            float v[8];
-            OpB::frag8(stage + OpA::BYTES, wn * TN * 32 + j * 32, kk, lane, v);
-            split_frag(v, bh[j], bl[j]);
-        }
+        for (int j = 0; j < TN; ++j) OpB::frag8(stage + OpA::BYTES, wn * TN * 32 + j * 32, kk, lane, rb[j]);
+    };
+    auto split = [&](const float(&ra)[TM][8], const float(&rb)[TN][8], bf16x8(&ah)[TM], bf16x8(&al)[TM],
+                     bf16x8(&bh)[TN], bf16x8(&bl)[TN]) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) split_frag(ra[i], ah[i], al[i]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) split_frag(rb[j], bh[j], bl[j]);
     };
     // small terms first, then the leading product (the register path's order)
     auto mma = [&](const bf16x8(&ah)[TM], const bf16x8(&al)[TM], const bf16x8(&bh)[TN], const bf16x8(&bl)[TN]) {
@@ -131,6 +135,21 @@ __device__ __forceinline__ void x3_tile_dma(const float* __restrict__ A, const f
 #pragma unroll
             for (int j = 0; j < TN; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+    };
+    // one matrix instruction, then a share of the split (and of the LDS
+    // reads and stage loads, if any), ...
+    constexpr int NMF = 3 * TM * TN;
+    constexpr int VPM = (3 * 8 * (TM + TN) + NMF - 1) / NMF;  // split instructions per MFMA
+    auto interleave = [&](bool reads_first) {
+        // the LDS reads of both steps of the next stage ahead of everything:
+        // the first products cover their latency
+        if (reads_first) __builtin_amdgcn_sched_group_barrier(0x100, 16 * (TM + TN), 0);
+#pragma unroll
+        for (int i = 0; i < NMF; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);    // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, VPM, 0);  // VALU
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);    // global load (to LDS)
+        }
     };
 
     constexpr int LOADS = OpA::PER_WAVE + OpB::PER_WAVE;  // loads per wave per stage
@@ -155,28 +174,36 @@ __device__ __forceinline__ void x3_tile_dma(const float* __restrict__ A, const f
         __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));
     }
     __builtin_amdgcn_s_barrier();
-    frags(sb, 0, ah0, al0, bh0, bl0);
+    frags(sb, 0, ra0, rb0);
+    frags(sb, 16, ra1, rb1);
+    split(ra0, rb0, ah0, al0, bh0, bl0);
 
-    // Two 16-deep steps per stage: the second step's fragments are read and
-    // split while the first step's products run; then each wave waits for
-    // the next stage and for its own reads of this one, the waves meet at a
-    // barrier, the loads of the stage after next go into this stage's
-    // buffer, and the next stage's first fragments are read under the second
-    // step's products.  Past the last stage the loads repeat it (unused), so
-    // every iteration has the same shape.
+    // Iteration t runs the two 16-deep steps of stage t.  First half: the
+    // products of step 0 while step 1's operands are split (its FP32
+    // values were read in the previous iteration).  Then each wave waits
+    // for stage t+1 and the waves meet at a barrier (every wave has read
+    // all of stage t by then); the loads of stage t+2 go into stage t's
+    // buffer.  Second half: both steps of stage t+1 are read from LDS, and
+    // the products of step 1 run while step 0 of stage t+1 is split.  Past
+    // the last stage the loads repeat it and the reads repeat the next
+    // buffer (unused), so every iteration has the same shape.
     for (int t = 0; t < KT; ++t) {
         char* cur = sb + (t & 1) * STAGE;
         const char* nxt = sb + ((t + 1) & 1) * STAGE;
         __builtin_amdgcn_sched_barrier(0);
-        frags(cur, 16, ah1, al1, bh1, bl1);
+        split(ra1, rb1, ah1, al1, bh1, bl1);
         mma(ah0, al0, bh0, bl0);
+        interleave(false);
         __builtin_amdgcn_sched_barrier(0);
-        __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the next stage in; this wave's reads of this one done
+        __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // stage t+1 in; no read of stage t in flight
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_sched_barrier(0);
         issue(t + 2 < KT ? t + 2 : KT - 1, cur);
-        frags(nxt, 0, ah0, al0, bh0, bl0);
+        frags(nxt, 0, ra0, rb0);
+        frags(nxt, 16, ra1, rb1);
+        split(ra0, rb0, ah0, al0, bh0, bl0);
         mma(ah1, al1, bh1, bl1);
+        interleave(true);
         __builtin_amdgcn_sched_barrier(0);
     }
     __builtin_amdgcn_s_waitcnt(waitcnt_imm(0, 0));  // the repeated loads land before LDS is reused
