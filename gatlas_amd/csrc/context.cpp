@@ -208,13 +208,14 @@ MfmaCfg Context::default_cfg(int dtype, const GemmArgs& a) {
         // that gives two work-groups per CU.  Interleaved timing in one
         // process, general form, against the two-group 64 x 64 tile:
         // 8192^3 NN 70.1 vs 62.6, TT 67.4 vs 63.6; 4096^3 NN 68.7 vs 61.8;
-        // 2048^3 NN 65.3 vs 57.7 TFLOP/s.  Smaller problems keep 64 x 64
-        // k-split tiles: four wave groups (32-deep stages) below two
-        // work-groups per CU, else two (16-deep).
+        // 2048^3 NN 65.3 vs 57.7 TFLOP/s.  Smaller problems take the 64 x 64
+        // tile of two k-split wave groups staged 16 deep, whose interior
+        // tiles stage straight into LDS: at 1024^3 the table timing puts it
+        // at 48.5-53.5 TFLOP/s over the four transpose pairs against
+        // 45.3-46.3 for four groups staged 32 deep through registers.
         const MfmaCfg wide = a.transA ? MfmaCfg{4, 2, 2, 2, 16, 16} : MfmaCfg{2, 4, 2, 2, 16, 16};
         if (tiles(wide) >= 2 * kFillTiles) return wide;
-        const MfmaCfg two{2, 2, 2, 2, 16, 16, 2};
-        return tiles(two) < 2 * kFillTiles ? MfmaCfg{2, 2, 2, 2, 32, 16, 4} : two;
+        return MfmaCfg{2, 2, 2, 2, 16, 16, 2};
     }
     const int64_t mn = a.M < a.N ? a.M : a.N;
     if (dtype == DT_FP8) {

@@ -7,8 +7,8 @@ work:
 Workloads (inputs U[-1,1] from the device generator):
   sgemm8192    C = 1.5*A*B - 0.5*C, FP32, M=N=K=8192, no transpose
   sgemm1024    C = A*B, FP32, M=N=K=1024, no transpose (default tile rule)
-  dgemm8192    the same in FP64
-  bf16x3_8192  split-BF16x3 (FP32 in/out), C = A*B
+  dgemm8192    the same in FP64;  dgemm1024: C = A*B, FP64, 1024^3
+  bf16x3_8192  split-BF16x3 (FP32 in/out), C = A*B;  bf16x3_2048 at 2048^3
   bf16_8192    BF16 inputs, FP32 out, C = A*B
   fp8_8192     resident e4m3 operands with block scales, C = A*B
   sgemv16384   y = A*x + y, FP32, 16384 x 16384 (A^T with sgemv16384t)
@@ -54,7 +54,9 @@ _GEMM_SHAPES = {
     "sgemm8192": (_capi.GAT_F32, 0, 0, 8192, 8192, 8192),
     "sgemm1024": (_capi.GAT_F32, 0, 0, 1024, 1024, 1024),
     "dgemm8192": (_capi.GAT_F64, 0, 0, 8192, 8192, 8192),
+    "dgemm1024": (_capi.GAT_F64, 0, 0, 1024, 1024, 1024),
     "bf16x3_8192": (_capi.GAT_BF16X3, 0, 0, 8192, 8192, 8192),
+    "bf16x3_2048": (_capi.GAT_BF16X3, 0, 0, 2048, 2048, 2048),
     "bf16_8192": (_capi.GAT_BF16, 0, 0, 8192, 8192, 8192),
     "fp8_8192": (_capi.GAT_FP8, 0, 1, 8192, 8192, 8192),
 }
@@ -115,11 +117,16 @@ def vendor(name):
 
 def run(ctx, name, reps):
     n = 8192
-    if name == "sgemm1024":
+    if name in ("sgemm1024", "dgemm1024"):
         m = 1024
-        A, B, C = _sq(ctx, m)
+        A, B, C = _sq(ctx, m, np.float32 if name[0] == "s" else np.float64)
         for _ in range(reps):
             ctx.gemm(A, B, C, m, m, m)
+    elif name == "bf16x3_2048":
+        m = 2048
+        A, B, C = _sq(ctx, m)
+        for _ in range(reps):
+            ctx.gemm(A, B, C, m, m, m, dtype=_capi.GAT_BF16X3)
     elif name in ("sgemm8192", "dgemm8192"):
         A, B, C = _sq(ctx, n, np.float32 if name[0] == "s" else np.float64)
         for _ in range(reps):
