@@ -15,32 +15,86 @@
 namespace gat {
 
 constexpr int X3S_ROWS = 64;  // rows per work-group of the split pass
+constexpr int X3S_K = 128;    // k values per work-group of the split pass
+
+// Eight consecutive k values of one row split into the planes: hi words to
+// d[0..3], lo words to d[16..19] (as dwords; the plane pair of a 32-k block
+// is 32 hi then 32 lo bf16).
+__device__ __forceinline__ void x3_split_store8(const float (&v)[8], __bf16* d) {
+    unsigned h[4], l[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) split_bf16x2(v[2 * j], v[2 * j + 1], h[j], l[j]);
+    gstore(d, u32x4{h[0], h[1], h[2], h[3]});
+    gstore(d + 32, u32x4{l[0], l[1], l[2], l[3]});
+}
 
 // dst row r (r < gridDim.y * 64): for each 32-k block b, 32 hi then 32 lo;
-// src element (r, k) = src[r*rs + k*cs], zero outside rows x K.
+// src element (r, k) = src[r*rs + k*cs], zero outside rows x K.  A
+// work-group splits 64 rows x 128 k values (k < kpad); work-item t takes
+// rows t/16 + 16i (i < 4) and the eight k values 8*(t%16) .. +7 of each, so
+// every 16 work-items write 512 contiguous bytes of a row's planes.
+// k-contiguous sources (cs == 1) are read straight into registers, 16 bytes
+// at a time where `vec`; row-contiguous ones (rs == 1) go through LDS,
+// 16 bytes of four rows per load, and are read back a row at a time.
 __global__ __launch_bounds__(256) void x3_split_kernel(const float* __restrict__ src, int64_t rs, int64_t cs,
-                                                      int rows, int K, int64_t ldd, __bf16* __restrict__ dst) {
-    __shared__ float tile[X3S_ROWS][33];
-    const int b = blockIdx.x, r0 = blockIdx.y * X3S_ROWS, tid = threadIdx.x;
-    const bool kfast = cs == 1;  // k-contiguous source: lanes walk k, else rows
-    for (int e = tid; e < X3S_ROWS * 32; e += 256) {
-        const int r = kfast ? e >> 5 : e & 63, k = kfast ? e & 31 : e >> 6;
-        const int gr = r0 + r, gk = b * 32 + k;
-        tile[r][k] = (gr < rows && gk < K) ? gload<float>(src + (int64_t)gr * rs + (int64_t)gk * cs) : 0.0f;
+                                                      int rows, int K, int kpad, int64_t ldd, int vec,
+                                                      __bf16* __restrict__ dst) {
+    __shared__ float tile[X3S_K][X3S_ROWS + 1];
+    const int k0 = blockIdx.x * X3S_K, r0 = blockIdx.y * X3S_ROWS, tid = threadIdx.x;
+    const int kk = k0 + 8 * (tid & 15);  // this work-item's first k value
+    if (cs == 1) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int gr = r0 + (tid >> 4) + 16 * i;
+            float v[8];
+            const float* p = src + (int64_t)gr * rs + kk;
+            if (vec && gr < rows && kk + 8 <= K) {
+                const f4v a = gload<f4v>(p), b = gload<f4v>(p + 4);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    v[e] = a[e];
+                    v[e + 4] = b[e];
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = (gr < rows && kk + e < K) ? gload<float>(p + e) : 0.0f;
+            }
+            if (kk < kpad) x3_split_store8(v, dst + (int64_t)gr * ldd + (kk >> 5) * 64 + (kk & 31));
+        }
+        return;
+    }
+    if (rs == 1) {
+        // 128 k-rows of 64 rows: 16 float4 per k-row, 8 per work-item
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = (tid >> 4) + 16 * i, rq = 4 * (tid & 15);
+            const int gk = k0 + k, gr = r0 + rq;
+            const float* p = src + (int64_t)gk * cs + gr;
+            if (vec && gk < K && gr + 4 <= rows) {
+                const f4v a = gload<f4v>(p);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) tile[k][rq + e] = a[e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) tile[k][rq + e] = (gk < K && gr + e < rows) ? gload<float>(p + e) : 0.0f;
+            }
+        }
+    } else {
+        for (int q = tid; q < X3S_K * X3S_ROWS; q += 256) {
+            const int k = q >> 6, r = q & 63, gk = k0 + k, gr = r0 + r;
+            tile[k][r] = (gk < K && gr < rows) ? gload<float>(src + (int64_t)gr * rs + (int64_t)gk * cs) : 0.0f;
+        }
     }
     __syncthreads();
-    // work-item: row tid/4; part tid%4 = hi k 0-15, hi 16-31, lo 0-15, lo 16-31
-    const int r = tid >> 2, p = tid & 3, kb = 16 * (p & 1);
-    unsigned wv[8];
+    if (kk >= kpad) return;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        unsigned hw, lw;
-        split_bf16x2(tile[r][kb + 2 * j], tile[r][kb + 2 * j + 1], hw, lw);
-        wv[j] = p < 2 ? hw : lw;
+    for (int i = 0; i < 4; ++i) {
+        const int r = (tid >> 4) + 16 * i;
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = tile[kk - k0 + e][r];
+        x3_split_store8(v, dst + (int64_t)(r0 + r) * ldd + (kk >> 5) * 64 + (kk & 31));
     }
-    __bf16* d = dst + (int64_t)(r0 + r) * ldd + b * 64 + (p < 2 ? 0 : 32) + kb;
-    gstore(d, u32x4{wv[0], wv[1], wv[2], wv[3]});
-    gstore(d + 8, u32x4{wv[4], wv[5], wv[6], wv[7]});
 }
 
 // The GEMM on split planes: As is Mp x 2*kpad, Bs is Np x 2*kpad (row
@@ -199,12 +253,15 @@ X3SplitDims x3_split_dims(int64_t M, int64_t N, int64_t K) {
 
 int launch_x3_split(const float* src, int64_t rs, int64_t cs, int rows, int K, int64_t rows_pad, int64_t kpad,
                     void* dst, hipStream_t st) {
-    if (rows_pad % X3S_ROWS || kpad % 32 || rows_pad / X3S_ROWS > 65535) {
+    if (rows_pad % X3S_ROWS || kpad % 32 || kpad > INT32_MAX || rows_pad / X3S_ROWS > 65535) {
         set_error("split-bf16x3: padded shape out of range");
         return ST_INVALID;
     }
-    hipLaunchKernelGGL(x3_split_kernel, dim3((unsigned)(kpad / 32), (unsigned)(rows_pad / X3S_ROWS)), dim3(256), 0,
-                       st, src, rs, cs, rows, K, 2 * kpad, (__bf16*)dst);
+    // 16-byte loads along the contiguous direction: aligned base and stride
+    const int64_t stride = cs == 1 ? rs : cs;
+    const int vec = ((uintptr_t)src % 16 == 0) && (stride % 4 == 0);
+    hipLaunchKernelGGL(x3_split_kernel, dim3((unsigned)((kpad + X3S_K - 1) / X3S_K), (unsigned)(rows_pad / X3S_ROWS)),
+                       dim3(256), 0, st, src, rs, cs, rows, K, (int)kpad, 2 * kpad, vec, (__bf16*)dst);
     GAT_HIP_TRY(hipGetLastError());
     return ST_OK;
 }
