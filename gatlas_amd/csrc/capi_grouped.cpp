@@ -129,8 +129,8 @@ int gat_grouped_tune(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t
             // shapes that tile this problem alike (e.g. 128 and 64 rows when
             // M <= 64) are timed once
             std::vector<std::pair<int, int>> rs, cs;
-            GroupedPlan::tile_spans((int64_t)d.M, gp_shape_bm(s), rs);
-            GroupedPlan::tile_spans((int64_t)d.N, gp_shape_bn(s), cs);
+            GroupedPlan::tile_spans((int64_t)d.M, gp_shape_bm(s), rs, gp_edge32(GroupedPlan::form_of(dtype)));
+            GroupedPlan::tile_spans((int64_t)d.N, gp_shape_bn(s), cs, gp_edge32(GroupedPlan::form_of(dtype)));
             bool dup = false;
             for (const auto& t : seen) dup = dup || (t.first == rs && t.second == cs);
             if (dup) continue;

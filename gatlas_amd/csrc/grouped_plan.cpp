@@ -26,11 +26,18 @@ MfmaCfg GroupedPlan::shape_cfg(int dtype, int shape) {
     return small ? MfmaCfg{2, 2, 1, 1, 32, 32} : MfmaCfg{2, 2, 2, 2, 32, 32};
 }
 
-void GroupedPlan::tile_spans(int64_t extent, int T, std::vector<std::pair<int, int>>& spans) {
+void GroupedPlan::tile_spans(int64_t extent, int T, std::vector<std::pair<int, int>>& spans, bool edge32) {
     spans.clear();
     for (int64_t o = 0; o < extent; o += T) {
         const int64_t rest = extent - o;
-        spans.emplace_back((int)o, (T == 128 && rest <= 64) ? 64 : T);
+        if (T != 128 || rest > 96 || (!edge32 && rest > 64)) {
+            spans.emplace_back((int)o, T);
+        } else if (rest <= 32 && edge32) {
+            spans.emplace_back((int)o, 32);
+        } else {
+            spans.emplace_back((int)o, 64);
+            if (rest > 64) spans.emplace_back((int)o + 64, 32);  // edge32 only
+        }
     }
 }
 
@@ -110,8 +117,8 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
         }
         rows.emplace_back();
         cols.emplace_back();
-        tile_spans(g.M, gp_shape_bm(s), rows.back());
-        tile_spans(g.N, gp_shape_bn(s), cols.back());
+        tile_spans(g.M, gp_shape_bm(s), rows.back(), gp_edge32(form_of(dtype)));
+        tile_spans(g.N, gp_shape_bn(s), cols.back(), gp_edge32(form_of(dtype)));
         const int tm = (int)rows.back().size(), tn = (int)cols.back().size();
         const double kc = (double)(g.K + 32);  // cost model: tile area times (K + 32)
         // A problem of many tiles is cut into up to GP_XCDS blocks of tiles
@@ -166,8 +173,12 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
             for (int r = u.r0; r < u.r1; ++r) {
                 const auto& rw = rows[u.p][r];
                 const auto& c = cols[u.p][ci];
-                lists[x].push_back(Tile{(double)rw.second * c.second * u.kcost, u.p, rw.first, c.first,
-                                        gp_shape_of(rw.second, c.second)});
+                // a 32-wide strip needs a 128-wide other side; else it is
+                // widened to 64 (it is the last strip: the extra rows or
+                // columns lie past the matrix and are not written)
+                const int bm = (rw.second == 32 && c.second != 128) ? 64 : rw.second;
+                const int bn = (c.second == 32 && rw.second != 128) ? 64 : c.second;
+                lists[x].push_back(Tile{(double)bm * bn * u.kcost, u.p, rw.first, c.first, gp_shape_of(bm, bn)});
             }
     }
     work.clear();

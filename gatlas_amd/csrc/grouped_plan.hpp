@@ -45,8 +45,8 @@ public:
     static int choose_shape(const Context& ctx, int dtype, const GroupedDesc& d);
     static MfmaCfg shape_cfg(int dtype, int shape);
     // (offset, extent) of the tiles of width T along a side of `extent`;
-    // with T = 128 a last tile over 64 or fewer elements is 64 wide
-    static void tile_spans(int64_t extent, int T, std::vector<std::pair<int, int>>& spans);
+    // with T = 128 the last rows are cut into strips of 64 and (edge32) 32
+    static void tile_spans(int64_t extent, int T, std::vector<std::pair<int, int>>& spans, bool edge32);
     // grouped-kernel precision form of a dtype (-1: not supported)
     static int form_of(int dtype);
     // Device descriptors and the cost-ordered (problem, m0, n0, shape) tile

@@ -18,7 +18,9 @@
 //
 // Tile shapes (grouped.hpp): 0 = 128 x 128 (2 x 2 waves of 2 x 2 32x32
 // blocks), 1 = 64 x 64 (one block per wave), 2 = 128 x 64 and 3 = 64 x 128
-// (2 x 1 and 1 x 2 blocks per wave); each work entry carries its tile's.
+// (2 x 1 and 1 x 2 blocks per wave), and for edge strips of the FP32 and
+// BF16 forms 4 = 128 x 32 and 5 = 32 x 128 (4 x 1 and 1 x 4 waves of one
+// block, staged through registers); each work entry carries its tile's.
 // BF16 interior tiles of problems with K >= 64 a multiple of 8, and FP32
 // and split-BF16x3 interior tiles of problems with K >= 32 a multiple of
 // 16, with 16-byte aligned rows, stage straight into LDS
@@ -42,77 +44,90 @@ namespace gat {
 // Interior FP32 tiles of problems with K >= 32 a multiple of 16 and
 // 16-byte aligned rows stage straight into LDS (sgemm_dma_tile.hpp, the
 // same bits).
-template <int TM, int TN, bool TA, bool TB>
+template <int WM, int WN, int TM, int TN, bool TA, bool TB>
 __device__ __forceinline__ void f32_tile_t(const GroupedProblemDev& p, int m0, int n0, float* smem) {
-    constexpr int BM = 2 * TM * 32, BN = 2 * TN * 32;
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     const bool vec = (p.flags & GP_VEC) != 0;
-    const bool dma = vec && p.K >= 32 && p.K % 16 == 0;
     const float* A = (const float*)p.A;
     const float* B = (const float*)p.B;
-    if (dma && m0 + BM <= p.M && n0 + BN <= p.N)
-        sgemm_tile_dma<2, 2, TM, TN, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0, n0,
-                                             smem);
-    else
-        sgemm_tile<2, 2, TM, TN, 32, 32, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0,
-                                                 n0, vec, smem);
+    if constexpr (BM >= 64 && BN >= 64) {
+        const bool dma = vec && p.K >= 32 && p.K % 16 == 0;
+        if (dma && m0 + BM <= p.M && n0 + BN <= p.N) {
+            sgemm_tile_dma<WM, WN, TM, TN, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
+                                                   m0, n0, smem);
+            return;
+        }
+    }
+    sgemm_tile<WM, WN, TM, TN, 32, 32, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0,
+                                               n0, vec, smem);
 }
 
 template <bool TA, bool TB>
 __device__ __forceinline__ void f32_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0, float* smem) {
-    if (shape == 0) f32_tile_t<2, 2, TA, TB>(p, m0, n0, smem);
-    else if (shape == 1) f32_tile_t<1, 1, TA, TB>(p, m0, n0, smem);
-    else if (shape == 2) f32_tile_t<2, 1, TA, TB>(p, m0, n0, smem);
-    else f32_tile_t<1, 2, TA, TB>(p, m0, n0, smem);
+    if (shape == 0) f32_tile_t<2, 2, 2, 2, TA, TB>(p, m0, n0, smem);
+    else if (shape == 1) f32_tile_t<2, 2, 1, 1, TA, TB>(p, m0, n0, smem);
+    else if (shape == 2) f32_tile_t<2, 2, 2, 1, TA, TB>(p, m0, n0, smem);
+    else if (shape == 3) f32_tile_t<2, 2, 1, 2, TA, TB>(p, m0, n0, smem);
+    else if (shape == 4) f32_tile_t<4, 1, 1, 1, TA, TB>(p, m0, n0, smem);
+    else f32_tile_t<1, 4, 1, 1, TA, TB>(p, m0, n0, smem);
 }
 
 // BF16 interior tiles of problems with K >= 64 a multiple of 8 and 16-byte
 // aligned rows stage straight into LDS; the others through registers, BK
 // deep.
-template <int TM, int TN, int BK, bool TA, bool TB>
+template <int WM, int WN, int TM, int TN, int BK, bool TA, bool TB>
 __device__ __forceinline__ void bf16_tile_t(const GroupedProblemDev& p, int m0, int n0, __bf16* smem) {
-    constexpr int BM = 2 * TM * 32, BN = 2 * TN * 32;
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     const bool vec = (p.flags & GP_VEC) != 0;
-    const bool dma = vec && p.K >= 64 && p.K % 8 == 0;
-    if (dma && m0 + BM <= p.M && n0 + BN <= p.N)
-        bf16_tile_dma<2, 2, TM, TN, TA, TB>((const __bf16*)p.A, (const __bf16*)p.B, p.C, p.M, p.N, p.K, p.lda,
-                                            p.ldb, p.ldc, p.alpha, p.beta, m0, n0, smem);
-    else
-        bf16_tile<2, 2, TM, TN, BK, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
-                                                       p.beta, m0, n0, vec, smem);
+    if constexpr (BM >= 64 && BN >= 64) {
+        const bool dma = vec && p.K >= 64 && p.K % 8 == 0;
+        if (dma && m0 + BM <= p.M && n0 + BN <= p.N) {
+            bf16_tile_dma<WM, WN, TM, TN, TA, TB>((const __bf16*)p.A, (const __bf16*)p.B, p.C, p.M, p.N, p.K, p.lda,
+                                                  p.ldb, p.ldc, p.alpha, p.beta, m0, n0, smem);
+            return;
+        }
+    }
+    bf16_tile<WM, WN, TM, TN, BK, MODE_BF16, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+                                                     p.beta, m0, n0, vec, smem);
 }
 
 template <bool TA, bool TB>
 __device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0,
                                                 __bf16* smem) {
-    if (shape == 0) bf16_tile_t<2, 2, 32, TA, TB>(p, m0, n0, smem);
-    else if (shape == 1) bf16_tile_t<1, 1, 64, TA, TB>(p, m0, n0, smem);
-    else if (shape == 2) bf16_tile_t<2, 1, 32, TA, TB>(p, m0, n0, smem);
-    else bf16_tile_t<1, 2, 32, TA, TB>(p, m0, n0, smem);
+    if (shape == 0) bf16_tile_t<2, 2, 2, 2, 32, TA, TB>(p, m0, n0, smem);
+    else if (shape == 1) bf16_tile_t<2, 2, 1, 1, 64, TA, TB>(p, m0, n0, smem);
+    else if (shape == 2) bf16_tile_t<2, 2, 2, 1, 32, TA, TB>(p, m0, n0, smem);
+    else if (shape == 3) bf16_tile_t<2, 2, 1, 2, 32, TA, TB>(p, m0, n0, smem);
+    else if (shape == 4) bf16_tile_t<4, 1, 1, 1, 64, TA, TB>(p, m0, n0, smem);
+    else bf16_tile_t<1, 4, 1, 1, 64, TA, TB>(p, m0, n0, smem);
 }
 
 // Split-BF16x3 interior tiles of problems with K >= 32 a multiple of 16 and
 // 16-byte aligned rows stage the FP32 operands straight into LDS and split
 // them as the fragments are read (x3_dma_tile.hpp, the same bits); the
 // others split while staging through registers, BK deep.
-template <int TM, int TN, int BK, bool TA, bool TB>
+template <int WM, int WN, int TM, int TN, int BK, bool TA, bool TB>
 __device__ __forceinline__ void x3_tile_t(const GroupedProblemDev& p, int m0, int n0, __bf16* smem) {
-    constexpr int BM = 2 * TM * 32, BN = 2 * TN * 32;
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     const bool vec = (p.flags & GP_VEC) != 0;
-    if (vec && p.K >= 32 && p.K % 16 == 0 && m0 + BM <= p.M && n0 + BN <= p.N)
-        x3_tile_dma<2, 2, TM, TN, TA, TB>((const float*)p.A, (const float*)p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb,
-                                          p.ldc, p.alpha, p.beta, m0, n0, (float*)smem);
-    else
-        bf16_tile<2, 2, TM, TN, BK, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
-                                                     p.beta, m0, n0, vec, smem);
+    if constexpr (x3_has_dma<WM, WN, TM, TN>()) {
+        if (vec && p.K >= 32 && p.K % 16 == 0 && m0 + BM <= p.M && n0 + BN <= p.N) {
+            x3_tile_dma<WM, WN, TM, TN, TA, TB>((const float*)p.A, (const float*)p.B, p.C, p.M, p.N, p.K, p.lda,
+                                                p.ldb, p.ldc, p.alpha, p.beta, m0, n0, (float*)smem);
+            return;
+        }
+    }
+    bf16_tile<WM, WN, TM, TN, BK, MODE_X3, TA, TB>(p.A, p.B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+                                                   p.beta, m0, n0, vec, smem);
 }
 
 template <bool TA, bool TB>
 __device__ __forceinline__ void x3_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0,
                                               __bf16* smem) {
-    if (shape == 0) x3_tile_t<2, 2, 16, TA, TB>(p, m0, n0, smem);
-    else if (shape == 1) x3_tile_t<1, 1, 32, TA, TB>(p, m0, n0, smem);
-    else if (shape == 2) x3_tile_t<2, 1, 16, TA, TB>(p, m0, n0, smem);
-    else x3_tile_t<1, 2, 16, TA, TB>(p, m0, n0, smem);
+    if (shape == 0) x3_tile_t<2, 2, 2, 2, 16, TA, TB>(p, m0, n0, smem);
+    else if (shape == 1) x3_tile_t<2, 2, 1, 1, 32, TA, TB>(p, m0, n0, smem);
+    else if (shape == 2) x3_tile_t<2, 2, 2, 1, 16, TA, TB>(p, m0, n0, smem);
+    else x3_tile_t<2, 2, 1, 2, 16, TA, TB>(p, m0, n0, smem);  // no 32-wide edge strips (gp_edge32)
 }
 
 // LDS of the tile buffers (the largest shape: 128 x 128; the 128 x 64
@@ -123,17 +138,17 @@ constexpr size_t gp_max(size_t a, size_t b) { return a > b ? a : b; }
 
 template <int FORM>
 constexpr size_t grouped_tile_bytes() {
-    // register-staged stages of shapes 0, 1 and 2 (3 takes what 2 does)
-    constexpr size_t reg = FORM == GP_FORM_F32
-                               ? gp_max(gp_max(SgemmShape<2, 2, 2, 2, 32, 32>::SMEM, SgemmShape<2, 2, 1, 1, 32, 32>::SMEM),
-                                        SgemmShape<2, 2, 2, 1, 32, 32>::SMEM)
-                           : FORM == GP_FORM_BF16
-                               ? gp_max(gp_max(Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM,
-                                               Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM),
-                                        Bf16Shape<2, 2, 2, 1, 32, MODE_BF16>::SMEM)
-                               : gp_max(gp_max(Bf16Shape<2, 2, 2, 2, 16, MODE_X3>::SMEM,
-                                               Bf16Shape<2, 2, 1, 1, 32, MODE_X3>::SMEM),
-                                        Bf16Shape<2, 2, 2, 1, 16, MODE_X3>::SMEM);
+    // register-staged stages of shapes 0, 1, 2 and 4 (3 and 5 take what 2
+    // and 4 do; the x3 form has no shapes 4 and 5)
+    constexpr size_t reg =
+        FORM == GP_FORM_F32
+            ? gp_max(gp_max(SgemmShape<2, 2, 2, 2, 32, 32>::SMEM, SgemmShape<2, 2, 1, 1, 32, 32>::SMEM),
+                     gp_max(SgemmShape<2, 2, 2, 1, 32, 32>::SMEM, SgemmShape<4, 1, 1, 1, 32, 32>::SMEM))
+        : FORM == GP_FORM_BF16
+            ? gp_max(gp_max(Bf16Shape<2, 2, 2, 2, 32, MODE_BF16>::SMEM, Bf16Shape<2, 2, 1, 1, 64, MODE_BF16>::SMEM),
+                     gp_max(Bf16Shape<2, 2, 2, 1, 32, MODE_BF16>::SMEM, Bf16Shape<4, 1, 1, 1, 64, MODE_BF16>::SMEM))
+            : gp_max(gp_max(Bf16Shape<2, 2, 2, 2, 16, MODE_X3>::SMEM, Bf16Shape<2, 2, 1, 1, 32, MODE_X3>::SMEM),
+                     Bf16Shape<2, 2, 2, 1, 16, MODE_X3>::SMEM);
     // direct-to-LDS stages (the 128 x 128 shape's are the largest)
     constexpr size_t dma = FORM == GP_FORM_BF16 ? Bf16DmaShape<2, 2, 2, 2>::SMEM
                            : FORM == GP_FORM_F32 ? SgemmDmaShape<2, 2, 2, 2>::SMEM

@@ -11,16 +11,23 @@ enum : int {
     GP_VEC = 4,  // 16-byte aligned operands and leading dimensions
 };
 
-// Tile shapes, all of four wavefronts (2 x 2, each wave TM x TN 32x32
-// blocks): 0 = 128 x 128, 1 = 64 x 64, 2 = 128 x 64, 3 = 64 x 128.  A
-// problem has one interior shape; where the last tile row or column holds
-// 64 or fewer rows (columns) of a 128-wide shape, that strip takes the
-// 64-wide shape of the same other side, so an edge computes at most 63
-// rows or columns past the matrix instead of 127.
-constexpr int GP_NSHAPES = 4;
-constexpr int gp_shape_bm(int s) { return (s == 0 || s == 2) ? 128 : 64; }
-constexpr int gp_shape_bn(int s) { return (s == 0 || s == 3) ? 128 : 64; }
-constexpr int gp_shape_of(int bm, int bn) { return bm == 128 ? (bn == 128 ? 0 : 2) : (bn == 128 ? 3 : 1); }
+// Tile shapes, all of four wavefronts of 32x32 blocks.  Interior shapes
+// (2 x 2 waves, each TM x TN blocks): 0 = 128 x 128, 1 = 64 x 64,
+// 2 = 128 x 64, 3 = 64 x 128; a problem has one.  Edge-only shapes: 4 =
+// 128 x 32 (4 x 1 waves of one block) and 5 = 32 x 128 (1 x 4 waves).
+// The last rows (columns) of a 128-wide shape are cut into the fewest
+// strips of 128, 64 and 32 that cover them to a multiple of 32 (at most 16
+// rows past the matrix for M a multiple of 16, against 48 with 64-wide
+// strips alone); a 32-wide strip stands next to a 128-wide side, and
+// where the other side is narrower it is widened to 64.
+constexpr int GP_NSHAPES = 4;      // interior shapes
+constexpr int GP_NSHAPES_ALL = 6;  // with the edge-only ones
+constexpr int gp_shape_bm(int s) { return s == 5 ? 32 : (s == 0 || s == 2 || s == 4) ? 128 : 64; }
+constexpr int gp_shape_bn(int s) { return s == 4 ? 32 : (s == 0 || s == 3 || s == 5) ? 128 : 64; }
+// shape of a bm x bn tile (a 32-wide side pairs with a 128-wide one)
+constexpr int gp_shape_of(int bm, int bn) {
+    return bm == 32 ? 5 : bn == 32 ? 4 : bm == 128 ? (bn == 128 ? 0 : 2) : (bn == 128 ? 3 : 1);
+}
 
 struct GroupedProblemDev {
     const void* A;
@@ -46,6 +53,13 @@ enum : int {
     GP_FORM_BF16 = 1,  // bf16 operands, FP32 accumulation
     GP_FORM_X3 = 2     // FP32 operands split into bf16 hi/lo while staged, 3 products
 };
+
+// Whether a form cuts 128-wide edges into 32-wide strips (shapes 4 and 5):
+// the split-BF16x3 routines of those strips raise the whole kernel's
+// register spills (18 -> 59) and cost more than the edge work they save
+// (224 -> 217 TFLOP/s on the 512-problem workload), so that form keeps the
+// 64-wide strips alone.
+constexpr bool gp_edge32(int form) { return form != GP_FORM_X3; }
 
 size_t grouped_smem_bytes(int form);
 // Resident work-groups per CU of a form's persistent kernel (LDS-limited).
