@@ -352,6 +352,10 @@ int gat_quantize_fp8(gat_ctx* ctx, const float* src, size_t rows, size_t cols, s
         set_error("quantize fp8: invalid operand");
         return GAT_ERR_INVALID;
     }
+    if (rows && cols && ld < (trans ? rows : cols)) {
+        set_error("quantize fp8: leading dimension shorter than the stored rows");
+        return GAT_ERR_INVALID;
+    }
     ctx->impl.make_current();
     // element (r, c) = trans ? src[c*ld + r] : src[r*ld + c]
     return launch_quantize_fp8(src, trans ? 1 : (int64_t)ld, trans ? (int64_t)ld : 1, (int)rows, (int)cols, out,

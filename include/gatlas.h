@@ -153,7 +153,8 @@ int gat_gemm_compat(gat_ctx *ctx, int dtype, int transA, int transB, size_t M, s
 int gat_convert_f32_bf16(gat_ctx *ctx, const float *src, void *dst, size_t n);
 /* FP8 e4m3 block quantisation of a rows x cols operand whose element (r, c) is
  * trans ? src[c*ld + r] : src[r*ld + c]: writes rows x cols bytes row-major and
- * ceil(rows/128) x ceil(cols/128) FP32 block scales (powers of two >= amax/448). */
+ * ceil(rows/128) x ceil(cols/128) FP32 block scales (powers of two >= amax/448).
+ * ld below cols (trans: below rows) is rejected with GAT_ERR_INVALID. */
 int gat_quantize_fp8(gat_ctx *ctx, const float *src, size_t rows, size_t cols, size_t ld, int trans, void *out,
                      float *scales);
 /* C = alpha * (A8 .* sA) (B8 .* sB)^T + beta*C with A8 = op(A) as M x K e4m3 and

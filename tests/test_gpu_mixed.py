@@ -66,6 +66,40 @@ def test_fp8_quantizer_matches_torch_cast(ctx):
     assert np.array_equal(qt.download().reshape(cols, rows), qh.reshape(rows, cols).T)
 
 
+def test_fp8_quantizer_short_leading_dimension_rejected(ctx):
+    x = ctx.to_device(uniform_host(64 * 64, 24, 0))
+    q, s = ctx.empty(64 * 64, np.uint8), ctx.empty(1, np.float32)
+    for trans in (0, 1):
+        rc = ctx.lib.gat_quantize_fp8(ctx.h, x.ptr, 64, 64, 63, trans, q.ptr, s.ptr)
+        assert rc == _capi.GAT_ERR_INVALID, (trans, rc)
+
+
+@pytest.mark.parametrize("rows,cols,ld", [(131, 257, 257), (129, 200, 203), (256, 384, 388)])
+def test_fp8_quantizer_ragged_and_strided(ctx, rows, cols, ld):
+    """Row lengths not a multiple of 4 (element loads, single-byte stores),
+    leading dimensions other than the row length, aligned or not, both read
+    directions: the bytes and scales of torch's float8_e4m3fn cast of x /
+    scale, and the transposed read gives the transposed layout."""
+    x = uniform_host(rows * ld, 23, 0) * np.float32(5.0)
+    x[7] = -0.0
+    X = x.reshape(rows, ld)[:, :cols]
+    dx = ctx.to_device(x)
+    cb = (cols + 127) // 128
+    nb = ((rows + 127) // 128) * cb
+    q, s = ctx.empty(rows * cols, np.uint8), ctx.empty(nb, np.float32)
+    ctx.quantize_fp8(dx, rows, cols, ld, False, q, s)
+    sh = s.download()
+    srep = np.repeat(np.repeat(sh.reshape(-1, cb), 128, axis=0)[:rows], 128, axis=1)[:, :cols]
+    ref = torch.from_numpy((X / srep).astype(np.float32)).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    assert np.array_equal(q.download().reshape(rows, cols), ref)
+    # the same matrix read transposed: element (c, r) of a cols x rows operand
+    rb = (rows + 127) // 128
+    qt, st = ctx.empty(rows * cols, np.uint8), ctx.empty(nb, np.float32)
+    ctx.quantize_fp8(dx, cols, rows, ld, True, qt, st)
+    assert np.array_equal(qt.download().reshape(cols, rows), ref.T)
+    assert np.array_equal(st.download().reshape(cb, rb), sh.reshape(rb, cb).T)
+
+
 def _operands(M, N, K, ta, tb, seed):
     A = uniform_host(M * K, seed, 0)
     B = uniform_host(K * N, seed, 1)
@@ -176,21 +210,27 @@ def test_bf16x3_direct_lds_path(ctx, ta, tb, K):
     rb, cb = (N, K) if tb else (K, N)
     dA, dB = ctx.to_device(A), ctx.to_device(B)
     pA, pB = ctx.to_device(_pad_rows(A, ra, ca, 1)), ctx.to_device(_pad_rows(B, rb, cb, 1))
+    # rows padded to a 16-byte multiple: still the direct-to-LDS path, with
+    # leading dimensions other than the row length
+    qA, qB = ctx.to_device(_pad_rows(A, ra, ca, 4)), ctx.to_device(_pad_rows(B, rb, cb, 4))
     C64, MAG = ora.gemm64(A, B, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, C0=C0)
     for cfg in ctx.tile_cfgs(_capi.GAT_BF16X3):
         if tuple(cfg[:6]) == SPLIT_X3:
             continue
         _say("bf16x3 direct-LDS", cfg, ta, tb, K)
-        d1, d2 = ctx.to_device(C0), ctx.to_device(C0)
+        d1, d2, d3 = ctx.to_device(C0), ctx.to_device(C0), ctx.to_device(C0)
         ctx.gemm(dA, dB, d1, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, dtype=_capi.GAT_BF16X3, cfg=cfg)
         ctx.gemm(pA, pB, d2, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, dtype=_capi.GAT_BF16X3,
                  lda=ca + 1, ldb=cb + 1, cfg=cfg)
-        got, ref = d1.download(), d2.download()
+        ctx.gemm(qA, qB, d3, M, N, K, transA=ta, transB=tb, alpha=1.25, beta=0.5, dtype=_capi.GAT_BF16X3,
+                 lda=ca + 4, ldb=cb + 4, cfg=cfg)
+        got, ref, got4 = d1.download(), d2.download(), d3.download()
         cw, _, _ = ora.error_metrics(got, C64, MAG)
         assert cw <= TOL, (cfg, ta, tb, K, cw)
         cw2, _, _ = ora.error_metrics(ref, C64, MAG)
         assert cw2 <= TOL, (cfg, ta, tb, K, cw2)
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (cfg, ta, tb, K)
+        assert np.array_equal(got4.view(np.uint32), ref.view(np.uint32)), (cfg, ta, tb, K, "padded rows")
 
 
 SPLIT_X3 = (2, 4, 4, 2, 32, 32)  # split-plane form of split-BF16x3
@@ -215,6 +255,16 @@ def test_bf16x3_split_planes_match_register_form(ctx, ta, tb):
     cw, _, _ = ora.error_metrics(got, C64, MAG)
     assert cw <= TOL, cw
     assert np.array_equal(got, ref)
+    # the split pass over rows padded to 16-byte multiples (vector loads) and
+    # by one element (element loads) gives the same planes
+    ra, ca = (K, M) if ta else (M, K)
+    rb, cb = (N, K) if tb else (K, N)
+    for pad in (4, 1):
+        pA, pB = ctx.to_device(_pad_rows(A, ra, ca, pad)), ctx.to_device(_pad_rows(B, rb, cb, pad))
+        d3 = ctx.to_device(C0)
+        ctx.gemm(pA, pB, d3, M, N, K, transA=ta, transB=tb, alpha=1.5, beta=-0.5, dtype=_capi.GAT_BF16X3,
+                 lda=ca + pad, ldb=cb + pad, cfg=SPLIT_X3)
+        assert np.array_equal(d3.download(), ref), pad
 
 
 def test_bf16x3_split_planes_strided_batch(ctx):
