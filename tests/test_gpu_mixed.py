@@ -446,6 +446,31 @@ def test_bf16x3_8192_sampled(ctx, dist):
         a.free()
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("dist", ["normal", "illscaled"])
+def test_bf16x3_2048_direct_lds_sampled(ctx, dist, ta, tb):
+    # 2048^3 takes the mid-size default: 128 x 128 tiles, every tile interior,
+    # on the direct-to-LDS x3 routine; general form, row sample against FP64.
+    n = 2048
+    _say("bf16x3 2048", dist, ta, tb)
+    dA, dB = ctx.empty(n * n), ctx.empty(n * n)
+    ctx.fill_normal(dA, 0x4D495846, 0)
+    ctx.fill_normal(dB, 0x4D495846, 1)
+    if dist == "illscaled":
+        ctx.scale_columns(dA, n, n, 0x4D495846, 10)
+        ctx.scale_columns(dB, n, n, 0x4D495846, 11)
+    C0 = uniform_host(n * n, 39, 2)
+    dC = ctx.to_device(C0)
+    ctx.gemm(dA, dB, dC, n, n, n, bool(ta), bool(tb), alpha=1.5, beta=-0.5, dtype=_capi.GAT_BF16X3)
+    rows = np.arange(0, n, 97)
+    got = dC.download().reshape(n, n)[rows]
+    A, B = dA.download(), dB.download()
+    C64, MAG = ora.gemm64(A, B, n, n, n, transA=ta, transB=tb, alpha=1.5, beta=-0.5, C0=C0, rows=rows)
+    cw, nw, _ = ora.error_metrics(got.ravel(), C64, MAG)
+    _say("bf16x3 2048 %s %d%d: componentwise %.3e normwise %.3e" % (dist, ta, tb, cw, nw))
+    assert cw <= TOL
+
+
 def test_fp8_resident_without_scales_rejected(ctx):
     # resident e4m3 operands without block scales: rejected on the host before
     # any launch, C untouched
