@@ -4,6 +4,25 @@
 
 namespace gat {
 
+// Tile rows per band when a problem's tiles are queued (an XCD's 64
+// resident groups then cover about an 8 x 8 square of tiles)
+constexpr int GP_BAND = 8;
+
+// Pairs of consecutive 128-wide spans become one 256-wide span.
+static void merge_pairs(std::vector<std::pair<int, int>>& spans) {
+    std::vector<std::pair<int, int>> out;
+    out.reserve(spans.size());
+    for (size_t i = 0; i < spans.size(); ++i) {
+        if (spans[i].second == 128 && i + 1 < spans.size() && spans[i + 1].second == 128) {
+            out.emplace_back(spans[i].first, 256);
+            ++i;
+        } else {
+            out.push_back(spans[i]);
+        }
+    }
+    spans.swap(out);
+}
+
 GroupedPlan::~GroupedPlan() {
     if (!ctx_) return;
     ctx_->make_current();
@@ -119,6 +138,10 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
         cols.emplace_back();
         tile_spans(g.M, gp_shape_bm(s), rows.back(), gp_edge32(form_of(dtype)));
         tile_spans(g.N, gp_shape_bn(s), cols.back(), gp_edge32(form_of(dtype)));
+        // BF16: pairs of whole 128 x 128 tiles along the longer side become
+        // one 256-wide tile where its direct-to-LDS routine applies
+        if (gp_merge256(form_of(dtype)) && s == 0 && vec && g.K >= 32 && g.K % 16 == 0)
+            merge_pairs(g.M >= g.N ? rows.back() : cols.back());
         const int tm = (int)rows.back().size(), tn = (int)cols.back().size();
         const double kc = (double)(g.K + 32);  // cost model: tile area times (K + 32)
         // A problem of many tiles is cut into up to GP_XCDS blocks of tiles
@@ -168,18 +191,35 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
         for (int y = 1; y < GP_XCDS; ++y)
             if (load[y] < load[x]) x = y;
         load[x] += u.cost;
-        // column-major inside the block: consecutive tiles share a B panel
-        for (int ci = u.c0; ci < u.c1; ++ci)
-            for (int r = u.r0; r < u.r1; ++r) {
-                const auto& rw = rows[u.p][r];
-                const auto& c = cols[u.p][ci];
-                // a 32-wide strip needs a 128-wide other side; else it is
-                // widened to 64 (it is the last strip: the extra rows or
-                // columns lie past the matrix and are not written)
-                const int bm = (rw.second == 32 && c.second != 128) ? 64 : rw.second;
-                const int bn = (c.second == 32 && rw.second != 128) ? 64 : c.second;
-                lists[x].push_back(Tile{(double)bm * bn * u.kcost, u.p, rw.first, c.first, gp_shape_of(bm, bn)});
-            }
+        auto emit = [&](int m0, int h, int n0, int wdt) {
+            // a 32-wide strip needs a 128-wide other side; else it is
+            // widened to 64 (it is the last strip: the extra rows or
+            // columns lie past the matrix and are not written)
+            const int bm = (h == 32 && wdt != 128) ? 64 : h;
+            const int bn = (wdt == 32 && h != 128) ? 64 : wdt;
+            lists[x].push_back(Tile{(double)bm * bn * u.kcost, u.p, m0, n0, gp_shape_of(bm, bn)});
+        };
+        // Inside the block, bands of GP_BAND tile rows, column-major in each
+        // band: consecutive tiles share a B panel, and the groups of one
+        // XCD (2 per CU x 32 CUs) working on a band at once read about
+        // GP_BAND A and GP_BAND B panels, not the block's whole height of A.
+        for (int rb = u.r0; rb < u.r1; rb += GP_BAND)
+            for (int ci = u.c0; ci < u.c1; ++ci)
+                for (int r = rb; r < std::min(rb + GP_BAND, u.r1); ++r) {
+                    const auto& rw = rows[u.p][r];
+                    const auto& c = cols[u.p][ci];
+                    // a 256-wide span pairs with a 128-wide one only; next
+                    // to an edge strip it is the two 128-wide tiles again
+                    if (rw.second == 256 && c.second != 128) {
+                        emit(rw.first, 128, c.first, c.second);
+                        emit(rw.first + 128, 128, c.first, c.second);
+                    } else if (c.second == 256 && rw.second != 128) {
+                        emit(rw.first, rw.second, c.first, 128);
+                        emit(rw.first, rw.second, c.first + 128, 128);
+                    } else {
+                        emit(rw.first, rw.second, c.first, c.second);
+                    }
+                }
     }
     work.clear();
     for (int x = 0; x < GP_XCDS; ++x) {

@@ -18,9 +18,11 @@
 //
 // Tile shapes (grouped.hpp): 0 = 128 x 128 (2 x 2 waves of 2 x 2 32x32
 // blocks), 1 = 64 x 64 (one block per wave), 2 = 128 x 64 and 3 = 64 x 128
-// (2 x 1 and 1 x 2 blocks per wave), and for edge strips of the FP32 and
+// (2 x 1 and 1 x 2 blocks per wave), for edge strips of the FP32 and
 // BF16 forms 4 = 128 x 32 and 5 = 32 x 128 (4 x 1 and 1 x 4 waves of one
-// block, staged through registers); each work entry carries its tile's.
+// block, staged through registers), and for the BF16 form the merged
+// 6 = 256 x 128 and 7 = 128 x 256 (4 x 2 and 2 x 4 blocks per wave, 32-deep
+// stages straight into LDS); each work entry carries its tile's.
 // BF16 interior tiles of problems with K >= 64 a multiple of 8, and FP32
 // and split-BF16x3 interior tiles of problems with K >= 32 a multiple of
 // 16, with 16-byte aligned rows, stage straight into LDS
@@ -40,6 +42,10 @@
 #include "x3_dma_tile.hpp"
 
 namespace gat {
+
+// LDS slots of the merged BF16 tiles' 32-deep stages (two: 48 KiB for the
+// 256 x 128 tile; three, 72 KiB, ran 6 % slower on the 512-problem workload)
+constexpr int GP_MERGED_SLOTS = 2;
 
 // Interior FP32 tiles of problems with K >= 32 a multiple of 16 and
 // 16-byte aligned rows stage straight into LDS (sgemm_dma_tile.hpp, the
@@ -91,6 +97,26 @@ __device__ __forceinline__ void bf16_tile_t(const GroupedProblemDev& p, int m0, 
                                                      p.beta, m0, n0, vec, smem);
 }
 
+// Merged BF16 tile (shape 6: TM = 4, 256 x 128; shape 7: TN = 4, 128 x 256):
+// 32-deep stages straight into LDS in GP_MERGED_SLOTS slots
+// (bf16_dma_tile.hpp).  The plan makes one only where that routine applies;
+// anywhere else it is the two 128 x 128 tiles it stands for, which give the
+// same bits.
+template <int TM, int TN, bool TA, bool TB>
+__device__ __forceinline__ void bf16_tile_merged(const GroupedProblemDev& p, int m0, int n0, __bf16* smem) {
+    constexpr int BM = 2 * TM * 32, BN = 2 * TN * 32;
+    const bool vec = (p.flags & GP_VEC) != 0;
+    if (vec && p.K >= 32 && p.K % 16 == 0 && m0 + BM <= p.M && n0 + BN <= p.N) {
+        bf16_tile_dma_ring<2, 2, TM, TN, GP_MERGED_SLOTS, TA, TB>((const __bf16*)p.A, (const __bf16*)p.B, p.C,
+                                                                  p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+                                                                  p.beta, m0, n0, smem);
+        return;
+    }
+    bf16_tile_t<2, 2, 2, 2, 32, TA, TB>(p, m0, n0, smem);
+    __syncthreads();  // every wave is done with the LDS before the second tile
+    bf16_tile_t<2, 2, 2, 2, 32, TA, TB>(p, TM == 4 ? m0 + 128 : m0, TM == 4 ? n0 : n0 + 128, smem);
+}
+
 template <bool TA, bool TB>
 __device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int shape, int m0, int n0,
                                                 __bf16* smem) {
@@ -99,7 +125,9 @@ __device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int 
     else if (shape == 2) bf16_tile_t<2, 2, 2, 1, 32, TA, TB>(p, m0, n0, smem);
     else if (shape == 3) bf16_tile_t<2, 2, 1, 2, 32, TA, TB>(p, m0, n0, smem);
     else if (shape == 4) bf16_tile_t<4, 1, 1, 1, 64, TA, TB>(p, m0, n0, smem);
-    else bf16_tile_t<1, 4, 1, 1, 64, TA, TB>(p, m0, n0, smem);
+    else if (shape == 5) bf16_tile_t<1, 4, 1, 1, 64, TA, TB>(p, m0, n0, smem);
+    else if (shape == 6) bf16_tile_merged<4, 2, TA, TB>(p, m0, n0, smem);
+    else bf16_tile_merged<2, 4, TA, TB>(p, m0, n0, smem);
 }
 
 // Split-BF16x3 interior tiles of problems with K >= 32 a multiple of 16 and
@@ -150,7 +178,8 @@ constexpr size_t grouped_tile_bytes() {
             : gp_max(gp_max(Bf16Shape<2, 2, 2, 2, 16, MODE_X3>::SMEM, Bf16Shape<2, 2, 1, 1, 32, MODE_X3>::SMEM),
                      Bf16Shape<2, 2, 2, 1, 16, MODE_X3>::SMEM);
     // direct-to-LDS stages (the 128 x 128 shape's are the largest)
-    constexpr size_t dma = FORM == GP_FORM_BF16 ? Bf16DmaShape<2, 2, 2, 2>::SMEM
+    constexpr size_t dma = FORM == GP_FORM_BF16 ? gp_max(Bf16DmaShape<2, 2, 2, 2>::SMEM,
+                                                         Bf16RingShape<2, 2, 4, 2, GP_MERGED_SLOTS>::SMEM)
                            : FORM == GP_FORM_F32 ? SgemmDmaShape<2, 2, 2, 2>::SMEM
                                                  : X3DmaShape<2, 2, 2, 2>::SMEM;
     return (gp_max(reg, dma) + 15) / 16 * 16;

@@ -20,13 +20,23 @@ enum : int {
 // rows past the matrix for M a multiple of 16, against 48 with 64-wide
 // strips alone); a 32-wide strip stands next to a 128-wide side, and
 // where the other side is narrower it is widened to 64.
+// Merged shapes of the BF16 form: 6 = 256 x 128 and 7 = 128 x 256 (2 x 2
+// waves of 4 x 2 and 2 x 4 blocks), two neighbouring whole 128 x 128 tiles
+// of a problem with K a multiple of 16 and aligned rows taken as one: a
+// quarter fewer LDS and L2 bytes per product, and half the per-tile cost.
 constexpr int GP_NSHAPES = 4;      // interior shapes
-constexpr int GP_NSHAPES_ALL = 6;  // with the edge-only ones
-constexpr int gp_shape_bm(int s) { return s == 5 ? 32 : (s == 0 || s == 2 || s == 4) ? 128 : 64; }
-constexpr int gp_shape_bn(int s) { return s == 4 ? 32 : (s == 0 || s == 3 || s == 5) ? 128 : 64; }
-// shape of a bm x bn tile (a 32-wide side pairs with a 128-wide one)
+constexpr int GP_NSHAPES_ALL = 8;  // with the edge-only and merged ones
+constexpr int gp_shape_bm(int s) {
+    return s == 6 ? 256 : s == 5 ? 32 : (s == 0 || s == 2 || s == 4 || s == 7) ? 128 : 64;
+}
+constexpr int gp_shape_bn(int s) {
+    return s == 7 ? 256 : s == 4 ? 32 : (s == 0 || s == 3 || s == 5 || s == 6) ? 128 : 64;
+}
+// shape of a bm x bn tile (a 32-wide side pairs with a 128-wide one, a
+// 256-wide side with a 128-wide one)
 constexpr int gp_shape_of(int bm, int bn) {
-    return bm == 32 ? 5 : bn == 32 ? 4 : bm == 128 ? (bn == 128 ? 0 : 2) : (bn == 128 ? 3 : 1);
+    return bm == 256 ? 6 : bn == 256 ? 7 : bm == 32 ? 5 : bn == 32 ? 4
+         : bm == 128 ? (bn == 128 ? 0 : 2) : (bn == 128 ? 3 : 1);
 }
 
 struct GroupedProblemDev {
@@ -60,6 +70,11 @@ enum : int {
 // (224 -> 217 TFLOP/s on the 512-problem workload), so that form keeps the
 // 64-wide strips alone.
 constexpr bool gp_edge32(int form) { return form != GP_FORM_X3; }
+// Whether a form merges pairs of 128 x 128 tiles into shapes 6 and 7 (the
+// BF16 form: its 128 x 128 tile is bound by LDS traffic; the FP32 one is
+// already bound by the matrix unit, and the split-BF16x3 routines of a
+// 256-wide tile would not fit the registers).
+constexpr bool gp_merge256(int form) { return form == GP_FORM_BF16; }
 
 size_t grouped_smem_bytes(int form);
 // Resident work-groups per CU of a form's persistent kernel (LDS-limited).

@@ -16,6 +16,7 @@ Workloads (inputs U[-1,1] from the device generator):
   grouped512   512 FP32 problems (M, N, K log-uniform in [64, 4096]) in one
                persistent grouped launch (untuned tile rule)
   grouped512x3 the same problems in split-BF16x3
+  grouped512bf16 the same sizes and transposes with bf16 operands (BF16 form)
 
 --tile W_M,W_N,T_M,T_N,BK,BLK,WK installs that tile for the GEMM workloads'
 size bucket first (the tuned-table entry the autotuner would write), so one
@@ -159,21 +160,30 @@ def run(ctx, name, reps):
             ctx.fill_uniform(a, 5, i)
         for _ in range(reps):
             ctx.gemv(A, x, y, m, m, transA=name.endswith("t"), alpha=1.0, beta=1.0)
-    elif name in ("grouped512", "grouped512x3"):
+    elif name in ("grouped512", "grouped512x3", "grouped512bf16"):
         # 512 problems, M, N, K log-uniform in [64, 4096] (multiples of 16),
         # random transposes, FP32; one persistent launch per repetition
         rng = np.random.default_rng(0x67525550)
         v = np.clip(np.round(np.exp(rng.uniform(np.log(64), np.log(4096), (512, 3))) / 16) * 16, 64,
                     4096).astype(int)
         tr = rng.integers(0, 2, (512, 2))
+        bf = name.endswith("bf16")
         probs = []
         for i in range(512):
             M, N, K = (int(x) for x in v[i])
             dA, dB, dC = ctx.empty(M * K), ctx.empty(K * N), ctx.empty(M * N)
             ctx.fill_uniform(dA, 21, 2 * i)
             ctx.fill_uniform(dB, 21, 2 * i + 1)
+            if bf:
+                hA, hB = ctx.empty(M * K, np.uint16), ctx.empty(K * N, np.uint16)
+                ctx.convert_bf16_n(dA, hA, M * K)
+                ctx.convert_bf16_n(dB, hB, K * N)
+                dA.free()
+                dB.free()
+                dA, dB = hA, hB
             probs.append(dict(transA=int(tr[i, 0]), transB=int(tr[i, 1]), M=M, N=N, K=K, A=dA, B=dB, C=dC))
-        g = ctx.grouped(probs, _capi.GAT_BF16X3 if name.endswith("x3") else _capi.GAT_F32, use_graph=False)
+        form = _capi.GAT_BF16X3 if name.endswith("x3") else _capi.GAT_BF16 if bf else _capi.GAT_F32
+        g = ctx.grouped(probs, form, use_graph=False)
         for _ in range(reps):
             g.run()
         ctx.sync()

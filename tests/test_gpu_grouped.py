@@ -5,7 +5,8 @@
     error <= 1e-5 (FP32 and split-BF16x3, against the FP32 operands) and
     <= 1e-5 against the bf16-rounded operands (BF16);
   * each problem's result is bit-identical to a single launch of the same
-    tile configuration (a problem computes the same alone as in the group);
+    tile configuration (a problem computes the same alone as in the group),
+    also on the BF16 form's merged 256 x 128 and 128 x 256 tiles;
   * a captured graph replayed several times, with eager work between the
     replays, gives the eager result every time;
   * 512 problems of the grouped workload (M, N, K log-uniform in [64, 4096],
@@ -119,6 +120,39 @@ def test_grouped_equals_single_launch(ctx, dtype):
         ctx.sync()
         single = p["C"].download()
         assert np.array_equal(grouped.view(np.uint32), single.view(np.uint32)), (M, N, K, ta, tb, s)
+
+
+def test_grouped_bf16_merged_tiles(ctx):
+    """BF16 problems with aligned rows and K a multiple of 16 run pairs of
+    whole 128 x 128 tiles as one 256 x 128 (M >= N) or 128 x 256 (N > M)
+    tile with 32-deep direct-to-LDS stages: K = 32 (one stage), 48 and 80
+    (a short first stage), 208 and 1040; all transpose pairs; sizes with an
+    odd number of whole 128-wide spans and 64- and 32-wide edge strips
+    beside the merged spans.  Each problem against the FP64 oracle of its
+    bf16-rounded operands and bit for bit against a single launch of its
+    shape's tile."""
+    dims = []
+    for i, K in enumerate((32, 48, 80, 208, 1040)):
+        for j, (ta, tb) in enumerate(((0, 0), (1, 0), (0, 1), (1, 1))):
+            M, N = ((400, 288), (304, 512), (384, 160), (256, 608))[(i + j) % 4]
+            dims.append((M, N, K, ta, tb))
+    _say("grouped bf16 merged", len(dims))
+    host, probs = _make_group(ctx, dims, 23, _capi.GAT_BF16)
+    g = ctx.grouped(probs, _capi.GAT_BF16, use_graph=True)
+    g.run()
+    ctx.sync()
+    shapes = g.shapes()
+    assert shapes.count(0) >= len(dims) // 2, shapes
+    worst = _check_oracle(dims, host, probs)
+    for (M, N, K, ta, tb), (A, B, C0, alpha, beta), p, s in zip(dims, host, probs, shapes):
+        grouped = p["C"].download()
+        p["C"].upload(C0)
+        ctx.gemm(p["A"], p["B"], p["C"], M, N, K, bool(ta), bool(tb), alpha, beta, dtype=_capi.GAT_BF16,
+                 cfg=SHAPE_CFG[_capi.GAT_BF16][s])
+        ctx.sync()
+        single = p["C"].download()
+        assert np.array_equal(grouped.view(np.uint32), single.view(np.uint32)), (M, N, K, ta, tb, s)
+    print("[grouped bf16 merged] worst componentwise err %.3e shapes %s tiles %s" % (worst, shapes, g.info()))
 
 
 def test_grouped_graph_replay(ctx):
