@@ -65,19 +65,33 @@ struct DmaOperand {
     // windows, so the 512 bytes of one read take two passes, not four.
     __device__ static __forceinline__ int tswz(int k) { return (4 * (k & 3)) & (CPR - 1); }
 
+    // Where a tile reaches past the last row (EDGE, rlim rows in all): a
+    // k-contiguous operand reads its last row there, a row-contiguous one
+    // (rlim a multiple of EPC) its last whole chunk, so every load stays
+    // inside the matrix and the rows past it hold copies.
+    template <bool EDGE>
+    __device__ static __forceinline__ int srow(int r, int rlim) {
+        return EDGE ? (r < rlim ? r : rlim - 1) : r;
+    }
+    template <bool EDGE>
+    __device__ static __forceinline__ int chunk_row(int r, int rlim) {
+        return EDGE ? (r < rlim - EPC ? r : rlim - EPC) : r;
+    }
+
     // Wavefront w fetches its blocks of the stage starting at k0 into `lds`.
+    template <bool EDGE = false>
     __device__ static __forceinline__ void issue(const T* src, int64_t ld, int r0, int k0, char* lds, int w,
-                                                 int lane) {
+                                                 int lane, int rlim = 0) {
 #pragma unroll
         for (int i = 0; i < PER_WAVE; ++i) {
             const int b = i * NW + w;
             const T* g;
             if constexpr (KCONTIG) {
                 const int row = RPB * b + lane / CH;
-                g = src + (int64_t)(r0 + row) * ld + k0 + EPC * ((lane % CH) ^ ksw(row));
+                g = src + (int64_t)srow<EDGE>(r0 + row, rlim) * ld + k0 + EPC * ((lane % CH) ^ ksw(row));
             } else {
                 const int k = b * KPB + lane / CPR;
-                g = src + (int64_t)(k0 + k) * ld + r0 + EPC * ((lane % CPR) ^ tswz(k));
+                g = src + (int64_t)(k0 + k) * ld + chunk_row<EDGE>(r0 + EPC * ((lane % CPR) ^ tswz(k)), rlim);
             }
             __builtin_amdgcn_global_load_lds((glb_void_t*)g, (lds_void_t*)(lds + b * 1024), 16, 0, 0);
         }
@@ -87,20 +101,22 @@ struct DmaOperand {
     // the rest of the KD zero) written into `lds` in this operand's layout
     // through registers, by all nt work-items: the first stage of a K range
     // that is not a multiple of KD.
+    template <bool EDGE = false>
     __device__ static __forceinline__ void stage_head(const T* src, int64_t ld, int r0, int k0, int klen, char* lds,
-                                                      int tid, int nt) {
+                                                      int tid, int nt, int rlim = 0) {
         if constexpr (KCONTIG) {
             for (int q = tid; q < ROWS * CH; q += nt) {
                 const int row = q / CH, c = q % CH;
                 u32x4 v = {0u, 0u, 0u, 0u};
-                if (EPC * c < klen) v = gload<u32x4>(src + (int64_t)(r0 + row) * ld + k0 + EPC * c);
+                if (EPC * c < klen)
+                    v = gload<u32x4>(src + (int64_t)srow<EDGE>(r0 + row, rlim) * ld + k0 + EPC * c);
                 *(u32x4*)(lds + row * RB + 16 * (c ^ ksw(row))) = v;
             }
         } else {
             for (int q = tid; q < KD * CPR; q += nt) {
                 const int k = q / CPR, c = q % CPR;
                 u32x4 v = {0u, 0u, 0u, 0u};
-                if (k < klen) v = gload<u32x4>(src + (int64_t)(k0 + k) * ld + r0 + EPC * c);
+                if (k < klen) v = gload<u32x4>(src + (int64_t)(k0 + k) * ld + chunk_row<EDGE>(r0 + EPC * c, rlim));
                 *(u32x4*)(lds + k * (ROWS * 2) + 16 * (c ^ tswz(k))) = v;
             }
         }
@@ -138,13 +154,15 @@ struct Bf16DmaShape {
 };
 
 // Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C for
-// a tile wholly inside M x N, K >= 64 a multiple of 8, 16-byte aligned rows;
-// all WM*WN*64 threads of the group call it; each wave returns with none
+// a tile wholly inside M x N (EDGE: any tile, with M a multiple of 8 when A
+// is transposed and N one when B is not -- rows past the matrix are loaded
+// as copies of its last rows and not stored), K >= 64 a multiple of 8,
+// 16-byte aligned rows; all WM*WN*64 threads of the group call it; each wave returns with none
 // of its own loads or LDS reads in flight (a caller that reuses the LDS
 // meets the other waves at a barrier first).  When K
 // is not a multiple of 64 the first stage holds the K % 64 leading k values
 // (staged through registers, zero-padded) and every later stage is whole.
-template <int WM, int WN, int TM, int TN, bool TA, bool TB>
+template <int WM, int WN, int TM, int TN, bool TA, bool TB, bool EDGE = false>
 __device__ __forceinline__ void bf16_tile_dma(const __bf16* __restrict__ A, const __bf16* __restrict__ B,
                                               float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb,
                                               int64_t ldc, float alpha, float beta, int m0, int n0, __bf16* smem) {
@@ -196,12 +214,12 @@ __device__ __forceinline__ void bf16_tile_dma(const __bf16* __restrict__ A, cons
 
     constexpr int LOADS = OpA::PER_WAVE + OpB::PER_WAVE;  // loads per wave per stage
     auto issue = [&](int st, char* buf) {
-        OpA::issue(A, lda, m0, kbase(st), buf, w, lane);
-        OpB::issue(B, ldb, n0, kbase(st), buf + OpA::BYTES, w, lane);
+        OpA::template issue<EDGE>(A, lda, m0, kbase(st), buf, w, lane, M);
+        OpB::template issue<EDGE>(B, ldb, n0, kbase(st), buf + OpA::BYTES, w, lane, N);
     };
     if (kh) {
-        OpA::stage_head(A, lda, m0, 0, kh, sb, tid, NW * 64);
-        OpB::stage_head(B, ldb, n0, 0, kh, sb + OpA::BYTES, tid, NW * 64);
+        OpA::template stage_head<EDGE>(A, lda, m0, 0, kh, sb, tid, NW * 64, M);
+        OpB::template stage_head<EDGE>(B, ldb, n0, 0, kh, sb + OpA::BYTES, tid, NW * 64, N);
     } else {
         issue(0, sb);
     }
@@ -256,6 +274,7 @@ __device__ __forceinline__ void bf16_tile_dma(const __bf16* __restrict__ A, cons
             for (int r = 0; r < 16; ++r) {
                 const int row = m0 + wm * TM * 32 + i * 32 + mfma32_row(lane, r);
                 const int col = n0 + wn * TN * 32 + j * 32 + mfma32_col(lane);
+                if (EDGE && (row >= M || col >= N)) continue;
                 float* p = C + (int64_t)row * ldc + col;
                 float v = alpha * acc[i][j][r];
                 if (beta != 0.0f) v += beta * gload<float>(p);

@@ -23,11 +23,12 @@
 // block, staged through registers), and for the BF16 form the merged
 // 6 = 256 x 128 and 7 = 128 x 256 (4 x 2 and 2 x 4 blocks per wave, 32-deep
 // stages straight into LDS); each work entry carries its tile's.
-// BF16 interior tiles of problems with K >= 64 a multiple of 8, and FP32
-// and split-BF16x3 interior tiles of problems with K >= 32 a multiple of
-// 16, with 16-byte aligned rows, stage straight into LDS
-// (bf16_dma_tile.hpp, sgemm_dma_tile.hpp, x3_dma_tile.hpp); the others
-// stage through registers.  The split-BF16x3 form splits FP32 operands into
+// Tiles of 64 and more on each side of problems with K >= 32 (BF16: 64) a
+// multiple of 16, 16-byte aligned rows and whole 16-byte chunks across each
+// row-contiguous operand stage straight into LDS, edge tiles included (rows
+// past the matrix are loaded as copies of its last rows and not stored;
+// bf16_dma_tile.hpp, sgemm_dma_tile.hpp, x3_dma_tile.hpp); the others stage
+// through registers.  The split-BF16x3 form splits FP32 operands into
 // bf16 hi/lo as each wave reads its fragments (direct-to-LDS tiles) or while
 // a stage is stored to LDS (bf16_tile.hpp, MODE_X3, staged 16 deep for the
 // 128-wide shapes, 32 for 64 x 64); two groups fit a CU.  Neither the
@@ -47,20 +48,27 @@ namespace gat {
 // 256 x 128 tile; three, 72 KiB, ran 6 % slower on the 512-problem workload)
 constexpr int GP_MERGED_SLOTS = 2;
 
-// Interior FP32 tiles of problems with K >= 32 a multiple of 16 and
-// 16-byte aligned rows stage straight into LDS (sgemm_dma_tile.hpp, the
-// same bits).
+// Whether a problem's tiles stage straight into LDS (edge tiles too: their
+// rows past the matrix are loaded as copies and not stored): 16-byte
+// aligned rows, K >= kmin a multiple of 16, and a whole number of 16-byte
+// chunks (e elements) across each row-contiguous operand.
+template <bool TA, bool TB>
+__device__ __forceinline__ bool gp_dma_ok(const GroupedProblemDev& p, int kmin, int e) {
+    return (p.flags & GP_VEC) != 0 && p.K >= kmin && p.K % 16 == 0 && (!TA || p.M % e == 0) &&
+           (TB || p.N % e == 0);
+}
+
+// FP32 tiles of 64 and more on each side stage straight into LDS where
+// gp_dma_ok holds (sgemm_dma_tile.hpp, the same bits as the register path).
 template <int WM, int WN, int TM, int TN, bool TA, bool TB>
 __device__ __forceinline__ void f32_tile_t(const GroupedProblemDev& p, int m0, int n0, float* smem) {
-    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     const bool vec = (p.flags & GP_VEC) != 0;
     const float* A = (const float*)p.A;
     const float* B = (const float*)p.B;
-    if constexpr (BM >= 64 && BN >= 64) {
-        const bool dma = vec && p.K >= 32 && p.K % 16 == 0;
-        if (dma && m0 + BM <= p.M && n0 + BN <= p.N) {
-            sgemm_tile_dma<WM, WN, TM, TN, TA, TB>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
-                                                   m0, n0, smem);
+    if constexpr (WM * TM >= 2 && WN * TN >= 2) {
+        if (gp_dma_ok<TA, TB>(p, 32, 4)) {
+            sgemm_tile_dma<WM, WN, TM, TN, TA, TB, 1, true>(A, B, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, p.alpha,
+                                                            p.beta, m0, n0, smem);
             return;
         }
     }
@@ -78,18 +86,17 @@ __device__ __forceinline__ void f32_tile_shape(const GroupedProblemDev& p, int s
     else f32_tile_t<1, 4, 1, 1, TA, TB>(p, m0, n0, smem);
 }
 
-// BF16 interior tiles of problems with K >= 64 a multiple of 8 and 16-byte
-// aligned rows stage straight into LDS; the others through registers, BK
-// deep.
+// BF16 tiles of 64 and more on each side stage straight into LDS where
+// gp_dma_ok holds (K >= 64; with K a multiple of 16 the routine's 16-deep
+// steps are the register path's, so the bits do not depend on the path);
+// the others through registers, BK deep.
 template <int WM, int WN, int TM, int TN, int BK, bool TA, bool TB>
 __device__ __forceinline__ void bf16_tile_t(const GroupedProblemDev& p, int m0, int n0, __bf16* smem) {
-    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     const bool vec = (p.flags & GP_VEC) != 0;
-    if constexpr (BM >= 64 && BN >= 64) {
-        const bool dma = vec && p.K >= 64 && p.K % 8 == 0;
-        if (dma && m0 + BM <= p.M && n0 + BN <= p.N) {
-            bf16_tile_dma<WM, WN, TM, TN, TA, TB>((const __bf16*)p.A, (const __bf16*)p.B, p.C, p.M, p.N, p.K, p.lda,
-                                                  p.ldb, p.ldc, p.alpha, p.beta, m0, n0, smem);
+    if constexpr (WM * TM >= 2 && WN * TN >= 2) {
+        if (gp_dma_ok<TA, TB>(p, 64, 8)) {
+            bf16_tile_dma<WM, WN, TM, TN, TA, TB, true>((const __bf16*)p.A, (const __bf16*)p.B, p.C, p.M, p.N, p.K,
+                                                        p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0, n0, smem);
             return;
         }
     }
@@ -130,18 +137,17 @@ __device__ __forceinline__ void bf16_tile_shape(const GroupedProblemDev& p, int 
     else bf16_tile_merged<2, 4, TA, TB>(p, m0, n0, smem);
 }
 
-// Split-BF16x3 interior tiles of problems with K >= 32 a multiple of 16 and
-// 16-byte aligned rows stage the FP32 operands straight into LDS and split
-// them as the fragments are read (x3_dma_tile.hpp, the same bits); the
-// others split while staging through registers, BK deep.
+// Split-BF16x3 tiles stage the FP32 operands straight into LDS where
+// gp_dma_ok holds and split them as the fragments are read (x3_dma_tile.hpp,
+// the same bits); the others split while staging through registers, BK
+// deep.
 template <int WM, int WN, int TM, int TN, int BK, bool TA, bool TB>
 __device__ __forceinline__ void x3_tile_t(const GroupedProblemDev& p, int m0, int n0, __bf16* smem) {
-    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     const bool vec = (p.flags & GP_VEC) != 0;
     if constexpr (x3_has_dma<WM, WN, TM, TN>()) {
-        if (vec && p.K >= 32 && p.K % 16 == 0 && m0 + BM <= p.M && n0 + BN <= p.N) {
-            x3_tile_dma<WM, WN, TM, TN, TA, TB>((const float*)p.A, (const float*)p.B, p.C, p.M, p.N, p.K, p.lda,
-                                                p.ldb, p.ldc, p.alpha, p.beta, m0, n0, (float*)smem);
+        if (gp_dma_ok<TA, TB>(p, 32, 4)) {
+            x3_tile_dma<WM, WN, TM, TN, TA, TB, true>((const float*)p.A, (const float*)p.B, p.C, p.M, p.N, p.K,
+                                                      p.lda, p.ldb, p.ldc, p.alpha, p.beta, m0, n0, (float*)smem);
             return;
         }
     }

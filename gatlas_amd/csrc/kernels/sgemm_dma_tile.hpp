@@ -45,14 +45,24 @@ struct DmaRowsF32 {
 
     __device__ static __forceinline__ int rsw(int k) { return 8 * ((k >> SB) & 1); }
 
+    // First row (column of the stored [k][row] matrix) of the 16-byte chunk
+    // at r; a tile that reaches past the last row (EDGE, rlim rows in all, a
+    // multiple of 4) fetches the last whole chunk instead, so every load
+    // stays inside the matrix and the rows past it hold copies.
+    template <bool EDGE>
+    __device__ static __forceinline__ int chunk_row(int r, int rlim) {
+        return EDGE ? (r < rlim - 4 ? r : rlim - 4) : r;
+    }
+
     // Wavefront w fetches its blocks of the stage starting at k0 into `lds`.
+    template <bool EDGE = false>
     __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
-                                                 int lane) {
+                                                 int lane, int rlim = 0) {
 #pragma unroll
         for (int i = 0; i < PER_WAVE; ++i) {
             const int b = i * NW + w;
             const int k = b * KPB + lane / CPR;
-            const float* g = src + (int64_t)(k0 + k) * ld + r0 + 4 * ((lane % CPR) ^ rsw(k));
+            const float* g = src + (int64_t)(k0 + k) * ld + chunk_row<EDGE>(r0 + 4 * ((lane % CPR) ^ rsw(k)), rlim);
             __builtin_amdgcn_global_load_lds((glb_void_t*)g, (lds_void_t*)(lds + b * 1024), 16, 0, 0);
         }
     }
@@ -60,13 +70,14 @@ struct DmaRowsF32 {
     // The same loads through per-lane addresses computed once per tile:
     // prep() gives the stage at k0 = 0, and the stage at k0 is then
     // issue_at(p, k0 * ld), one 64-bit add per load.
+    template <bool EDGE = false>
     __device__ static __forceinline__ void prep(const float* src, int64_t ld, int r0, int w, int lane,
-                                                const float* (&p)[PER_WAVE]) {
+                                                const float* (&p)[PER_WAVE], int rlim = 0) {
 #pragma unroll
         for (int i = 0; i < PER_WAVE; ++i) {
             const int b = i * NW + w;
             const int k = b * KPB + lane / CPR;
-            p[i] = src + (int64_t)k * ld + r0 + 4 * ((lane % CPR) ^ rsw(k));
+            p[i] = src + (int64_t)k * ld + chunk_row<EDGE>(r0 + 4 * ((lane % CPR) ^ rsw(k)), rlim);
         }
     }
     __device__ static __forceinline__ void issue_at(const float* const (&p)[PER_WAVE], int64_t off, char* lds,
@@ -81,13 +92,14 @@ struct DmaRowsF32 {
     // The k values [k0, k0 + klen) (klen < 32 a multiple of 4) written into
     // the last klen k-rows of a stage in `lds`, the first 32 - klen zero,
     // through registers by all nt work-items: a short first stage.
+    template <bool EDGE = false>
     __device__ static __forceinline__ void stage_head(const float* src, int64_t ld, int r0, int k0, int klen,
-                                                      char* lds, int tid, int nt) {
+                                                      char* lds, int tid, int nt, int rlim = 0) {
         for (int q = tid; q < 32 * CPR; q += nt) {
             const int k = q / CPR, c = q % CPR;
             const int kr = k - (32 - klen);  // k value held by k-row k, < 0: padding
             f4v v = {0.0f, 0.0f, 0.0f, 0.0f};
-            if (kr >= 0) v = gload<f4v>(src + (int64_t)(k0 + kr) * ld + r0 + 4 * c);
+            if (kr >= 0) v = gload<f4v>(src + (int64_t)(k0 + kr) * ld + chunk_row<EDGE>(r0 + 4 * c, rlim));
             *(f4v*)(lds + k * (ROWS * 4) + 16 * (c ^ rsw(k))) = v;
         }
     }
@@ -125,25 +137,35 @@ struct DmaColsF32 {
 
     __device__ static __forceinline__ int kswz(int r) { return SW ? (r >> 1) & 7 : r & 7; }
 
+    // Stored row of tile row r0 + row; a tile that reaches past the last
+    // row (EDGE, rlim rows in all) reads the last row there instead, so
+    // every load stays inside the matrix.
+    template <bool EDGE>
+    __device__ static __forceinline__ int srow(int r, int rlim) {
+        return EDGE ? (r < rlim ? r : rlim - 1) : r;
+    }
+
+    template <bool EDGE = false>
     __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
-                                                 int lane) {
+                                                 int lane, int rlim = 0) {
 #pragma unroll
         for (int i = 0; i < PER_WAVE; ++i) {
             const int b = i * NW + w;
             const int row = 8 * b + (lane >> 3);
-            const float* g = src + (int64_t)(r0 + row) * ld + k0 + 4 * ((lane & 7) ^ kswz(row));
+            const float* g = src + (int64_t)srow<EDGE>(r0 + row, rlim) * ld + k0 + 4 * ((lane & 7) ^ kswz(row));
             __builtin_amdgcn_global_load_lds((glb_void_t*)g, (lds_void_t*)(lds + b * 1024), 16, 0, 0);
         }
     }
 
     // per-lane addresses computed once per tile (see DmaRowsF32::prep); the
     // stage at k0 is issue_at(p, k0)
+    template <bool EDGE = false>
     __device__ static __forceinline__ void prep(const float* src, int64_t ld, int r0, int w, int lane,
-                                                const float* (&p)[PER_WAVE]) {
+                                                const float* (&p)[PER_WAVE], int rlim = 0) {
 #pragma unroll
         for (int i = 0; i < PER_WAVE; ++i) {
             const int row = 8 * (i * NW + w) + (lane >> 3);
-            p[i] = src + (int64_t)(r0 + row) * ld + 4 * ((lane & 7) ^ kswz(row));
+            p[i] = src + (int64_t)srow<EDGE>(r0 + row, rlim) * ld + 4 * ((lane & 7) ^ kswz(row));
         }
     }
     __device__ static __forceinline__ void issue_at(const float* const (&p)[PER_WAVE], int64_t off, char* lds,
@@ -157,13 +179,14 @@ struct DmaColsF32 {
 
     // short first stage through registers, k values in the last klen
     // positions of each row (see DmaRowsF32::stage_head)
+    template <bool EDGE = false>
     __device__ static __forceinline__ void stage_head(const float* src, int64_t ld, int r0, int k0, int klen,
-                                                      char* lds, int tid, int nt) {
+                                                      char* lds, int tid, int nt, int rlim = 0) {
         for (int q = tid; q < ROWS * 8; q += nt) {
             const int row = q >> 3, c = q & 7;
             const int kr = 4 * c - (32 - klen);  // first k value of chunk c, < 0: padding
             f4v v = {0.0f, 0.0f, 0.0f, 0.0f};
-            if (kr >= 0) v = gload<f4v>(src + (int64_t)(r0 + row) * ld + k0 + kr);
+            if (kr >= 0) v = gload<f4v>(src + (int64_t)srow<EDGE>(r0 + row, rlim) * ld + k0 + kr);
             *(f4v*)(lds + row * 128 + 16 * (c ^ kswz(row))) = v;
         }
     }
@@ -200,16 +223,18 @@ template <int ROWS, int NW, int SW>
 struct DmaF32Operand<ROWS, true, NW, SW> {
     typedef DmaColsF32<ROWS, NW, SW> Op;
     static constexpr int BYTES = Op::BYTES, PER_WAVE = Op::PER_WAVE;
+    template <bool EDGE = false>
     __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
-                                                 int lane) {
-        Op::issue(src, ld, r0, k0, lds, w, lane);
+                                                 int lane, int rlim = 0) {
+        Op::template issue<EDGE>(src, ld, r0, k0, lds, w, lane, rlim);
     }
     __device__ static __forceinline__ void frag(const char* lds, int rbase, int kk, int lane, float (&v)[4]) {
         Op::frag(lds, rbase, kk, lane, v);
     }
+    template <bool EDGE = false>
     __device__ static __forceinline__ void stage_head(const float* src, int64_t ld, int r0, int k0, int klen,
-                                                      char* lds, int tid, int nt) {
-        Op::stage_head(src, ld, r0, k0, klen, lds, tid, nt);
+                                                      char* lds, int tid, int nt, int rlim = 0) {
+        Op::template stage_head<EDGE>(src, ld, r0, k0, klen, lds, tid, nt, rlim);
     }
 };
 
@@ -217,16 +242,18 @@ template <int ROWS, int NW, int SW>
 struct DmaF32Operand<ROWS, false, NW, SW> {
     typedef DmaRowsF32<ROWS, NW> Op;
     static constexpr int BYTES = Op::BYTES, PER_WAVE = Op::PER_WAVE;
+    template <bool EDGE = false>
     __device__ static __forceinline__ void issue(const float* src, int64_t ld, int r0, int k0, char* lds, int w,
-                                                 int lane) {
-        Op::issue(src, ld, r0, k0, lds, w, lane);
+                                                 int lane, int rlim = 0) {
+        Op::template issue<EDGE>(src, ld, r0, k0, lds, w, lane, rlim);
     }
     __device__ static __forceinline__ void frag(const char* lds, int rbase, int kk, int lane, float (&v)[4]) {
         Op::frag(lds, rbase, kk, lane, v);
     }
+    template <bool EDGE = false>
     __device__ static __forceinline__ void stage_head(const float* src, int64_t ld, int r0, int k0, int klen,
-                                                      char* lds, int tid, int nt) {
-        Op::stage_head(src, ld, r0, k0, klen, lds, tid, nt);
+                                                      char* lds, int tid, int nt, int rlim = 0) {
+        Op::template stage_head<EDGE>(src, ld, r0, k0, klen, lds, tid, nt, rlim);
     }
 };
 
@@ -239,9 +266,11 @@ struct SgemmDmaShape {
 };
 
 // Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C for
-// a tile wholly inside M x N, K >= 32 a multiple of 16, 16-byte aligned
-// operand rows; all WM*WN*WK*64 threads of the group call it; ends after a
-// barrier.  When K is an odd multiple of 16 the first stage holds the 16
+// a tile wholly inside M x N (EDGE: any tile, with M a multiple of 4 when A
+// is transposed and N one when B is not -- rows past the matrix are loaded
+// as copies of its last rows and not stored), K >= 32 a multiple of 16,
+// 16-byte aligned operand rows; all WM*WN*WK*64 threads of the group call
+// it; ends after a barrier.  When K is an odd multiple of 16 the first stage holds the 16
 // leading k values in its second half (staged through registers, the first
 // half zero) and every later stage is whole, so stage s covers k
 // [32s - 16, 32s + 16).  The MFMAs pair the same k values as the register
@@ -251,7 +280,7 @@ struct SgemmDmaShape {
 // order, and the two partial sums commute: the bits agree.  WK > 1 splits each stage's four 8-deep steps between WK groups
 // of WM x WN waves as the register-staged k-split tile does (group g takes
 // steps g*4/WK ..), with the same partial-sum exchange, so the bits agree.
-template <int WM, int WN, int TM, int TN, bool TA, bool TB, int WK = 1>
+template <int WM, int WN, int TM, int TN, bool TA, bool TB, int WK = 1, bool EDGE = false>
 __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, const float* __restrict__ B,
                                                float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb,
                                                int64_t ldc, float alpha, float beta, int m0, int n0, float* smem) {
@@ -315,12 +344,12 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
 
     constexpr int LOADS = OpA::PER_WAVE + OpB::PER_WAVE;  // loads per wave per stage
     auto issue = [&](int st, char* buf) {
-        OpA::issue(A, lda, m0, kbase(st), buf, w, lane);
-        OpB::issue(B, ldb, n0, kbase(st), buf + OpA::BYTES, w, lane);
+        OpA::template issue<EDGE>(A, lda, m0, kbase(st), buf, w, lane, M);
+        OpB::template issue<EDGE>(B, ldb, n0, kbase(st), buf + OpA::BYTES, w, lane, N);
     };
     if (kh) {
-        OpA::stage_head(A, lda, m0, 0, kh, sb, threadIdx.x, NW * 64);
-        OpB::stage_head(B, ldb, n0, 0, kh, sb + OpA::BYTES, threadIdx.x, NW * 64);
+        OpA::template stage_head<EDGE>(A, lda, m0, 0, kh, sb, threadIdx.x, NW * 64, M);
+        OpB::template stage_head<EDGE>(B, ldb, n0, 0, kh, sb + OpA::BYTES, threadIdx.x, NW * 64, N);
     } else {
         issue(0, sb);
     }
@@ -376,8 +405,8 @@ __device__ __forceinline__ void sgemm_tile_dma(const float* __restrict__ A, cons
     // fragment reads must be done
     if constexpr (WK > 1) __syncthreads();
 
-    sgemm_finish<WM, WN, TM, TN, 32, WK, false>(acc, C, M, N, ldc, alpha, beta, m0, n0, wm, wn, wmn, wk, lane,
-                                                smem);
+    sgemm_finish<WM, WN, TM, TN, 32, WK, EDGE>(acc, C, M, N, ldc, alpha, beta, m0, n0, wm, wn, wmn, wk, lane,
+                                               smem);
     if constexpr (WK == 1) __syncthreads();  // LDS free for the caller's next tile
 }
 

@@ -70,10 +70,12 @@ __device__ __forceinline__ void split_frag(const float (&v)[8], bf16x8& hi, bf16
 }
 
 // Computes the BM x BN tile at (m0, n0) of C = alpha*op(A)op(B) + beta*C for
-// a tile wholly inside M x N, FP32 operands, K >= 32 a multiple of 16,
-// 16-byte aligned operand rows; all WM*WN*64 threads of the group call it;
-// ends after a barrier (LDS free for the caller's next tile).
-template <int WM, int WN, int TM, int TN, bool TA, bool TB>
+// a tile wholly inside M x N (EDGE: any tile, with M a multiple of 4 when A
+// is transposed and N one when B is not -- rows past the matrix are loaded
+// as copies of its last rows and not stored), FP32 operands, K >= 32 a
+// multiple of 16, 16-byte aligned operand rows; all WM*WN*64 threads of the
+// group call it; ends after a barrier (LDS free for the caller's next tile).
+template <int WM, int WN, int TM, int TN, bool TA, bool TB, bool EDGE = false>
 __device__ __forceinline__ void x3_tile_dma(const float* __restrict__ A, const float* __restrict__ B,
                                             float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb,
                                             int64_t ldc, float alpha, float beta, int m0, int n0, float* smem) {
@@ -155,15 +157,15 @@ __device__ __forceinline__ void x3_tile_dma(const float* __restrict__ A, const f
     constexpr int LOADS = OpA::PER_WAVE + OpB::PER_WAVE;  // loads per wave per stage
     const float* pa[OpA::PER_WAVE];
     const float* pb[OpB::PER_WAVE];
-    OpA::prep(A, lda, m0, w, lane, pa);
-    OpB::prep(B, ldb, n0, w, lane, pb);
+    OpA::template prep<EDGE>(A, lda, m0, w, lane, pa, M);
+    OpB::template prep<EDGE>(B, ldb, n0, w, lane, pb, N);
     auto issue = [&](int st, char* buf) {
         OpA::issue_at(pa, OpA::stage_off(kbase(st), lda), buf, w);
         OpB::issue_at(pb, OpB::stage_off(kbase(st), ldb), buf + OpA::BYTES, w);
     };
     if (kh) {
-        OpA::stage_head(A, lda, m0, 0, kh, sb, threadIdx.x, NW * 64);
-        OpB::stage_head(B, ldb, n0, 0, kh, sb + OpA::BYTES, threadIdx.x, NW * 64);
+        OpA::template stage_head<EDGE>(A, lda, m0, 0, kh, sb, threadIdx.x, NW * 64, M);
+        OpB::template stage_head<EDGE>(B, ldb, n0, 0, kh, sb + OpA::BYTES, threadIdx.x, NW * 64, N);
     } else {
         issue(0, sb);
     }
@@ -216,6 +218,7 @@ __device__ __forceinline__ void x3_tile_dma(const float* __restrict__ A, const f
             for (int r = 0; r < 16; ++r) {
                 const int row = m0 + wm * TM * 32 + i * 32 + mfma32_row(lane, r);
                 const int col = n0 + wn * TN * 32 + j * 32 + mfma32_col(lane);
+                if (EDGE && (row >= M || col >= N)) continue;
                 float* p = C + (int64_t)row * ldc + col;
                 float v = alpha * acc[i][j][r];
                 if (beta != 0.0f) v += beta * gload<float>(p);

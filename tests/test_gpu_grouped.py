@@ -155,6 +155,40 @@ def test_grouped_bf16_merged_tiles(ctx):
     print("[grouped bf16 merged] worst componentwise err %.3e shapes %s tiles %s" % (worst, shapes, g.info()))
 
 
+@pytest.mark.parametrize("dtype", FORMS, ids=FORM_IDS)
+def test_grouped_edge_tiles_direct_lds(ctx, dtype):
+    """Edge tiles of problems with K a multiple of 16 and aligned rows stage
+    straight into LDS, rows past the matrix loaded as copies of its last
+    rows and not stored: M and N multiples of 4 (BF16: 8) but not of 32,
+    edges 4 to 60 wide, K = 48 (a short first stage), 64 and 272, all
+    transpose pairs, general form.  Each problem against the FP64 oracle and
+    bit for bit against a single launch of its shape's tile, whose edge
+    tiles stage through registers."""
+    mult = 8 if dtype == _capi.GAT_BF16 else 4
+    sizes = [(196, 132), (72, 260), (140, 68), (324, 100)]
+    dims = []
+    for i, K in enumerate((64, 272) if dtype == _capi.GAT_BF16 else (48, 64, 272)):
+        for j, (ta, tb) in enumerate(((0, 0), (1, 0), (0, 1), (1, 1))):
+            M, N = sizes[(i + j) % 4]
+            dims.append((M // mult * mult, N // mult * mult, K, ta, tb))
+    _say("grouped edge tiles", dtype, len(dims))
+    host, probs = _make_group(ctx, dims, 29, dtype)
+    g = ctx.grouped(probs, dtype, use_graph=False)
+    g.run()
+    ctx.sync()
+    shapes = g.shapes()
+    worst = _check_oracle(dims, host, probs)
+    for (M, N, K, ta, tb), (A, B, C0, alpha, beta), p, s in zip(dims, host, probs, shapes):
+        grouped = p["C"].download()
+        p["C"].upload(C0)
+        ctx.gemm(p["A"], p["B"], p["C"], M, N, K, bool(ta), bool(tb), alpha, beta, dtype=dtype,
+                 cfg=SHAPE_CFG[dtype][s])
+        ctx.sync()
+        single = p["C"].download()
+        assert np.array_equal(grouped.view(np.uint32), single.view(np.uint32)), (M, N, K, ta, tb, s)
+    print("[grouped edge] dtype %d worst componentwise err %.3e shapes %s" % (dtype, worst, shapes))
+
+
 def test_grouped_graph_replay(ctx):
     rng = np.random.default_rng(9)
     dims = _ragged_dims(rng, 20, 8, 400)
