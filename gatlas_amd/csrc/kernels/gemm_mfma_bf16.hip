@@ -3,7 +3,9 @@
 // (bf16_tile.hpp), shared with the persistent grouped kernel.  Interior
 // tiles stage straight into LDS where a direct-to-LDS routine exists: BF16
 // inputs at BK = 64 (bf16_dma_tile.hpp), split-BF16x3 at any BK
-// (x3_dma_tile.hpp, 32-deep stages); both give the register path's bits.
+// (x3_dma_tile.hpp, 32-deep stages); both give the register path's bits
+// (BF16: for K a multiple of 16; for K = 8 mod 16 the two paths pad the
+// short stage at different ends and so group the k values differently).
 #include "bf16_dma_tile.hpp"
 #include "bf16_tile.hpp"
 #include "x3_dma_tile.hpp"

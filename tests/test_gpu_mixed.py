@@ -149,7 +149,7 @@ def _pad_rows(x, rows, cols, extra):
     return out
 
 
-@pytest.mark.parametrize("K", [192, 200])
+@pytest.mark.parametrize("K", [192, 200, 208])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_bf16_direct_lds_path(ctx, ta, tb, K):
     # M, N with whole and partial 256-wide tiles: the interior tiles of the
@@ -157,11 +157,12 @@ def test_bf16_direct_lds_path(ctx, ta, tb, K):
     # (k-contiguous operands XOR-swizzled, row-contiguous ones read back
     # with the transposing LDS read), the edge tiles through registers;
     # K = 200 adds a short first stage of 8 k values staged through
-    # registers.  Checked against the FP64 oracle, and for K a multiple of
-    # 64 bit for bit against the register path on the same data (leading
-    # dimensions one element longer, not 16-byte multiples, force that path
-    # on every tile; with a short first stage the two paths add the k
-    # stages in different groupings).
+    # registers (K = 208: a short first stage of 16).  Checked against the
+    # FP64 oracle, and for K a multiple of 16 bit for bit against the
+    # register path on the same data (leading dimensions one element longer,
+    # not 16-byte multiples, force that path on every tile; for K = 8 mod 16
+    # the two paths pad the short stage at different ends and so add the k
+    # values in different groupings).
     M, N = 512, 384
     A, B, C0 = _operands(M, N, K, ta, tb, 35)
     dA, dB = ctx.to_device(A), ctx.to_device(B)
@@ -188,7 +189,7 @@ def test_bf16_direct_lds_path(ctx, ta, tb, K):
         assert cw <= TOL, (cfg, ta, tb, cw)
         cw2, _, _ = ora.error_metrics(ref, C64, MAG)
         assert cw2 <= TOL, (cfg, ta, tb, cw2)
-        if K % 64 == 0:
+        if K % 16 == 0:
             assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (cfg, ta, tb)
 
 
