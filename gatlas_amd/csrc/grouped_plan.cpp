@@ -8,12 +8,14 @@ namespace gat {
 // resident groups then cover about an 8 x 8 square of tiles)
 constexpr int GP_BAND = 8;
 
-// Pairs of consecutive 128-wide spans become one 256-wide span.
-static void merge_pairs(std::vector<std::pair<int, int>>& spans) {
+// Pairs of consecutive 128-wide spans wholly inside a side of `extent`
+// become one 256-wide span.
+static void merge_pairs(std::vector<std::pair<int, int>>& spans, int64_t extent) {
     std::vector<std::pair<int, int>> out;
     out.reserve(spans.size());
     for (size_t i = 0; i < spans.size(); ++i) {
-        if (spans[i].second == 128 && i + 1 < spans.size() && spans[i + 1].second == 128) {
+        if (spans[i].second == 128 && i + 1 < spans.size() && spans[i + 1].second == 128 &&
+            spans[i].first + 256 <= extent) {
             out.emplace_back(spans[i].first, 256);
             ++i;
         } else {
@@ -141,7 +143,7 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
         // BF16: pairs of whole 128 x 128 tiles along the longer side become
         // one 256-wide tile where its direct-to-LDS routine applies
         if (gp_merge256(form_of(dtype)) && s == 0 && vec && g.K >= 32 && g.K % 16 == 0)
-            merge_pairs(g.M >= g.N ? rows.back() : cols.back());
+            merge_pairs(g.M >= g.N ? rows.back() : cols.back(), g.M >= g.N ? g.M : g.N);
         const int tm = (int)rows.back().size(), tn = (int)cols.back().size();
         const double kc = (double)(g.K + 32);  // cost model: tile area times (K + 32)
         // A problem of many tiles is cut into up to GP_XCDS blocks of tiles
@@ -208,12 +210,14 @@ int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* 
                 for (int r = rb; r < std::min(rb + GP_BAND, u.r1); ++r) {
                     const auto& rw = rows[u.p][r];
                     const auto& c = cols[u.p][ci];
-                    // a 256-wide span pairs with a 128-wide one only; next
-                    // to an edge strip it is the two 128-wide tiles again
-                    if (rw.second == 256 && c.second != 128) {
+                    // a 256-wide span pairs with a whole 128-wide one only;
+                    // next to an edge tile it is the two 128-wide tiles again
+                    const bool whole_c = c.second == 128 && c.first + 128 <= d[u.p].N;
+                    const bool whole_r = rw.second == 128 && rw.first + 128 <= d[u.p].M;
+                    if (rw.second == 256 && !whole_c) {
                         emit(rw.first, 128, c.first, c.second);
                         emit(rw.first + 128, 128, c.first, c.second);
-                    } else if (c.second == 256 && rw.second != 128) {
+                    } else if (c.second == 256 && !whole_r) {
                         emit(rw.first, rw.second, c.first, 128);
                         emit(rw.first, rw.second, c.first + 128, 128);
                     } else {
