@@ -35,7 +35,8 @@ constexpr int bf16_waves_per_simd() {
     constexpr int by_lds = groups * WM * WN / 4;
     constexpr int by_regs = 512 / (TM * TN * 16 + (MODE == MODE_X3 ? 160 : 96));
     constexpr int w = by_lds < by_regs ? by_lds : by_regs;
-    return w < 1 ? 1 : (w > 8 ? 8 : w);
+    constexpr int need = (WM * WN + 3) / 4;  // the group's own waves per SIMD
+    return w < need ? need : (w > 8 ? 8 : w);
 }
 
 template <int WM, int WN, int TM, int TN, int BK, int MODE, bool TA, bool TB>
@@ -87,6 +88,7 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_simd<WM, WN, TM, TN, BK, MODE>
     X(1, 1, 2, 2, 64, 3)  \
     X(2, 4, 4, 2, 32, 1)  \
     X(2, 4, 4, 2, 64, 1)  \
+    X(4, 4, 2, 2, 64, 1)  \
     X(2, 4, 4, 2, 16, 2)  \
     X(2, 2, 2, 2, 16, 2)  \
     X(2, 2, 1, 1, 32, 2)
