@@ -221,8 +221,11 @@ gat_grouped *gat_grouped_create(gat_ctx *ctx, int dtype, const gat_gemm_desc *de
 int gat_grouped_run(gat_grouped *g);
 int gat_grouped_info(gat_grouped *g, size_t *tiles, double *flops, int *grid);
 /* Interior tile shape of problem i: 0 = 128x128, 1 = 64x64, 2 = 128x64,
- * 3 = 64x128 (M x N); a last tile row or column over 64 or fewer rows or
- * columns of a 128-wide shape is computed 64 wide. */
+ * 3 = 64x128 (M x N).  The last rows (columns) of a 128-wide shape run as
+ * strips of 64 and, for dtype f32 and bf16, 32; for bf16, pairs of whole
+ * 128x128 tiles of a problem with K a multiple of 16 and 16-byte aligned
+ * rows run as one 256x128 or 128x256 tile.  None of this changes a
+ * problem's bits (the same single-launch tile computes them). */
 int gat_grouped_shape(gat_grouped *g, size_t i);
 void gat_grouped_destroy(gat_grouped *g);
 /* create + run + synchronise + destroy. */
