@@ -93,6 +93,8 @@ SIGNATURES = {
     "gat_grouped_run": (_c_int, [_c_vp]),
     "gat_grouped_info": (_c_int, [_c_vp, ctypes.POINTER(_c_sz), ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_int)]),
     "gat_grouped_shape": (_c_int, [_c_vp, _c_sz]),
+    "gat_grouped_plan": (ctypes.c_long, [_c_int, ctypes.POINTER(GemmDesc), _c_sz, ctypes.POINTER(_c_int),
+                                         ctypes.POINTER(_c_int), _c_sz]),
     "gat_grouped_destroy": (None, [_c_vp]),
     "gat_gemm_grouped": (_c_int, [_c_vp, _c_int, ctypes.POINTER(GemmDesc), _c_sz]),
     "gat_grouped_tune": (_c_int, [_c_vp, _c_int, ctypes.POINTER(GemmDesc), _c_sz, _c_int, ctypes.c_char_p]),

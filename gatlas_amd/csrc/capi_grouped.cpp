@@ -29,6 +29,29 @@ static std::vector<GroupedDesc> to_descs(const gat_gemm_desc* d, size_t n) {
 
 extern "C" {
 
+long gat_grouped_plan(int dtype, const gat_gemm_desc* descs, size_t count, const int* shapes, int* out,
+                      size_t cap) {
+    if ((count && !descs) || (cap && !out)) {
+        set_error("gat_grouped_plan: null descriptors or output");
+        return GAT_ERR_INVALID;
+    }
+    const std::vector<GroupedDesc> v = to_descs(descs, count);
+    std::vector<GroupedProblemDev> probs;
+    std::vector<int4> work;
+    GroupedQueues q{};
+    std::vector<int> used;
+    double flops = 0;
+    if (GroupedPlan::build_tables(nullptr, dtype, v.data(), v.size(), probs, work, q, used, flops, shapes) != ST_OK)
+        return GAT_ERR_INVALID;
+    for (size_t i = 0; i < work.size() && i < cap; ++i) {
+        out[4 * i] = work[i].x;
+        out[4 * i + 1] = work[i].y;
+        out[4 * i + 2] = work[i].z;
+        out[4 * i + 3] = work[i].w;
+    }
+    return (long)work.size();
+}
+
 gat_grouped* gat_grouped_create(gat_ctx* ctx, int dtype, const gat_gemm_desc* descs, size_t count, int use_graph) {
     if (!ctx || (count && !descs)) {
         set_error("gat_grouped_create: null context or descriptors");

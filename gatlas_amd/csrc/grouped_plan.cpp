@@ -71,7 +71,7 @@ int GroupedPlan::form_of(int dtype) {
     }
 }
 
-int GroupedPlan::choose_shape(const Context& ctx, int dtype, const GroupedDesc& d) {
+int GroupedPlan::choose_shape(const Context* ctx, int dtype, const GroupedDesc& d) {
     GemmArgs a;
     a.transA = d.transA;
     a.transB = d.transB;
@@ -79,12 +79,12 @@ int GroupedPlan::choose_shape(const Context& ctx, int dtype, const GroupedDesc& 
     a.N = d.N;
     a.K = d.K;
     int s = 0;
-    if (ctx.get_grouped_shape(bucket_of(dtype, a), &s)) return s;
+    if (ctx && ctx->get_grouped_shape(bucket_of(dtype, a), &s)) return s;
     // untuned: 128-wide sides where a side fills two of them, else 64
     return gp_shape_of(d.M >= 256 ? 128 : 64, d.N >= 256 ? 128 : 64);
 }
 
-int GroupedPlan::build_tables(const Context& ctx, int dtype, const GroupedDesc* d, size_t n,
+int GroupedPlan::build_tables(const Context* ctx, int dtype, const GroupedDesc* d, size_t n,
                               std::vector<GroupedProblemDev>& probs, std::vector<int4>& work, GroupedQueues& q,
                               std::vector<int>& shapes, double& flops, const int* force_shapes) {
     if (form_of(dtype) < 0) {
@@ -241,7 +241,7 @@ int GroupedPlan::create(Context& ctx, int dtype, const GroupedDesc* d, size_t n,
                         const int* force_shapes) {
     std::vector<GroupedProblemDev> probs;
     std::vector<int4> work;
-    int rc = build_tables(ctx, dtype, d, n, probs, work, queues_, shapes_, flops_, force_shapes);
+    int rc = build_tables(&ctx, dtype, d, n, probs, work, queues_, shapes_, flops_, force_shapes);
     if (rc) return rc;
     ctx_ = &ctx;
     form_ = form_of(dtype);

@@ -41,8 +41,8 @@ public:
     int shape(size_t i) const { return shapes_[i]; }
 
     // shape for one problem: the context's tuned choice for its size bucket
-    // when one exists, else a size rule
-    static int choose_shape(const Context& ctx, int dtype, const GroupedDesc& d);
+    // when one exists (ctx may be null), else a size rule
+    static int choose_shape(const Context* ctx, int dtype, const GroupedDesc& d);
     static MfmaCfg shape_cfg(int dtype, int shape);
     // (offset, extent) of the tiles of width T along a side of `extent`;
     // with T = 128 the last rows are cut into strips of 64 and (edge32) 32
@@ -51,8 +51,8 @@ public:
     static int form_of(int dtype);
     // Device descriptors and the cost-ordered (problem, m0, n0, shape) tile
     // list for n problems; also the interior shape per problem and the total
-    // flops.
-    static int build_tables(const Context& ctx, int dtype, const GroupedDesc* d, size_t n,
+    // flops.  Host work only: ctx (may be null) supplies tuned shapes.
+    static int build_tables(const Context* ctx, int dtype, const GroupedDesc* d, size_t n,
                             std::vector<GroupedProblemDev>& probs, std::vector<int4>& work, GroupedQueues& q,
                             std::vector<int>& shapes, double& flops, const int* force_shapes = nullptr);
 

@@ -212,7 +212,7 @@ int Server::create(Context& ctx, const gat_serve_cfg& cfg) {
         }
         if (r.kind == 0) {
             GroupedDesc d{r.ta, r.tb, r.M, r.N, r.K, 1.0, nullptr, r.lda, nullptr, r.ldb, 0.0, nullptr, r.N};
-            r.shape = GroupedPlan::choose_shape(ctx, cfg_.gemm_dtype, d);
+            r.shape = GroupedPlan::choose_shape(&ctx, cfg_.gemm_dtype, d);
         }
     }
     GAT_HIP_TRY(hipMalloc((void**)&pool_, pool_elems_ * sizeof(float)));
@@ -304,7 +304,7 @@ int Server::launch_batch(int kind, std::vector<int>& pend, int slot, Launch& L) 
         std::vector<int> used;
         double flops = 0;
         GroupedQueues q{};
-        int rc = GroupedPlan::build_tables(*ctx_, cfg_.gemm_dtype, descs.data(), descs.size(), probs, work, q, used,
+        int rc = GroupedPlan::build_tables(ctx_, cfg_.gemm_dtype, descs.data(), descs.size(), probs, work, q, used,
                                            flops, shapes.data());
         if (rc) return rc;
         // [descriptors | tile list] in one staging buffer, one copy

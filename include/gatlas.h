@@ -219,6 +219,17 @@ gat_grouped *gat_grouped_create(gat_ctx *ctx, int dtype, const gat_gemm_desc *de
                                 int use_graph);
 /* One persistent launch (a graph replay when created with use_graph). */
 int gat_grouped_run(gat_grouped *g);
+/* Host-only dry run of gat_grouped_create's schedule, no device or context
+ * needed (no counterpart in the reference: for callers that inspect or test
+ * a grouped launch's tile list).  Interior shapes from `shapes` (count
+ * entries, gat_grouped_shape numbering), or the untuned size rule when
+ * NULL.  Writes up to cap entries {problem, m0, n0, tile shape} to out
+ * (4 ints each) in launch order, the XCD queues one after the other, and
+ * returns the number of tiles, or GAT_ERR_INVALID.  Tile shapes 4 and 5 are
+ * 128x32 and 32x128 edge strips, 6 and 7 the merged bf16 256x128 and
+ * 128x256 tiles; operand pointers are only tested for 16-byte alignment. */
+long gat_grouped_plan(int dtype, const gat_gemm_desc *descs, size_t count, const int *shapes, int *out,
+                      size_t cap);
 int gat_grouped_info(gat_grouped *g, size_t *tiles, double *flops, int *grid);
 /* Interior tile shape of problem i: 0 = 128x128, 1 = 64x64, 2 = 128x64,
  * 3 = 64x128 (M x N).  The last rows (columns) of a 128-wide shape run as
