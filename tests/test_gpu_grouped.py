@@ -7,6 +7,8 @@
   * each problem's result is bit-identical to a single launch of the same
     tile configuration (a problem computes the same alone as in the group),
     also on the BF16 form's merged 256 x 128 and 128 x 256 tiles;
+  * the host-only dry run (gat_grouped_plan) given a launch's interior
+    shapes has as many tiles as that launch;
   * a captured graph replayed several times, with eager work between the
     replays, gives the eager result every time;
   * 512 problems of the grouped workload (M, N, K log-uniform in [64, 4096],
@@ -187,6 +189,24 @@ def test_grouped_edge_tiles_direct_lds(ctx, dtype):
         single = p["C"].download()
         assert np.array_equal(grouped.view(np.uint32), single.view(np.uint32)), (M, N, K, ta, tb, s)
     print("[grouped edge] dtype %d worst componentwise err %.3e shapes %s" % (dtype, worst, shapes))
+
+
+@pytest.mark.parametrize("dtype", FORMS, ids=FORM_IDS)
+def test_grouped_plan_matches_launch(ctx, dtype):
+    """gat_grouped_plan (host-only dry run) with a launch's own interior
+    shapes gives the tile list that launch runs: the same number of tiles
+    as gat_grouped_info reports, on real device operands."""
+    rng = np.random.default_rng(0x67525550)
+    dims = _workload_dims(rng, 48)
+    _say("grouped plan vs launch", dtype)
+    host, probs = _make_group(ctx, dims, 31, dtype)
+    g = ctx.grouped(probs, dtype, use_graph=False)
+    shapes = g.shapes()
+    descs = ctx.gemm_descs(probs)
+    sh = (ctypes.c_int * len(dims))(*shapes)
+    n = ctx.lib.gat_grouped_plan(dtype, descs, len(dims), sh, None, 0)
+    assert n == g.info()["tiles"], (n, g.info())
+    g.close()
 
 
 def test_grouped_graph_replay(ctx):
